@@ -1,0 +1,138 @@
+/*
+ * C ABI of the MI355X block-sparse tensor library (libquantit_amd.so).
+ *
+ * The entry points are the operations a foreign-function binding of QuantiT's
+ * block-sparse DMRG path calls: btensor construction and inspection, the
+ * block contraction, the block SVD/eigh with truncation and the DMRG driver.
+ * Plain pointers and sizes only.  Every function returns a status
+ * (QTT_OK = 0); on failure qtt_last_error() returns the message of the
+ * exception the reference would have thrown, and the status names its type.
+ * Block data lives in the device arena of a context; tensors of one context
+ * are used from one host thread at a time.
+ */
+#ifndef QUANTIT_AMD_H
+#define QUANTIT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QTT_OK 0
+#define QTT_INVALID_ARGUMENT 1 /* std::invalid_argument */
+#define QTT_OUT_OF_RANGE 2     /* std::out_of_range */
+#define QTT_DOMAIN_ERROR 3     /* std::domain_error */
+#define QTT_LOGIC_ERROR 4      /* std::logic_error */
+#define QTT_TORCH_CHECK 5      /* c10::Error from TORCH_CHECK */
+#define QTT_OUT_OF_MEMORY 6    /* device arena exhausted */
+#define QTT_HIP_ERROR 7        /* HIP runtime failure */
+#define QTT_RUNTIME_ERROR 8
+
+#define QTT_PRECISION_FP64 0       /* FP64 MFMA contractions */
+#define QTT_PRECISION_SPLIT_FP32 1 /* hi/lo split FP32 MFMA, FP64 recombination */
+
+typedef struct qtt_context qtt_context;
+typedef struct qtt_btensor qtt_btensor;
+
+/* Message of the last failed call on this thread. */
+const char* qtt_last_error(void);
+int qtt_version(void);
+
+/* ---- context: device, HIP stream and HBM arena ------------------------- */
+int qtt_context_create(int device, size_t arena_bytes, qtt_context** out);
+int qtt_context_destroy(qtt_context* ctx);
+int qtt_context_set_precision(qtt_context* ctx, int precision);
+/* 1: one GEMM launch per block pair (reference-shaped baseline) */
+int qtt_context_set_per_block(qtt_context* ctx, int enable);
+int qtt_context_set_profiling(qtt_context* ctx, int enable);
+int qtt_context_sync(qtt_context* ctx);
+int qtt_context_memory(qtt_context* ctx, size_t* used, size_t* peak, size_t* capacity);
+/* Writes "name ms launches\n" lines of the accumulated per-kernel times. */
+int qtt_context_kernel_times(qtt_context* ctx, char* buf, size_t buflen, int reset);
+
+/* ---- btensor ------------------------------------------------------------
+ * Replaces btensor(index_list sections_by_dim, any_quantity_vector c_vals,
+ * index_list section_sizes, any_quantity sel_rule)
+ * (include/blockTensor/btensor.h:128-133) plus the filling of its blocks.
+ * nsec[rank]: sections per dimension; sec_sizes[sum nsec]; cvals[sum nsec *
+ * ngroups] and sel[ngroups]: charges as int16 tuples; group_mod[ngroups]: 0
+ * for Z, N for C<N>.  block_idx[nblocks * rank] lists the blocks in
+ * lexicographic order; data holds them concatenated, each row-major. */
+int qtt_btensor_create(qtt_context* ctx, int rank, const int64_t* nsec, const int64_t* sec_sizes, int ngroups,
+                       const uint16_t* group_mod, const int16_t* cvals, const int16_t* sel, int64_t nblocks,
+                       const int32_t* block_idx, const double* data, qtt_btensor** out);
+int qtt_btensor_destroy(qtt_btensor* t);
+/* rank, number of groups, total section count, block count, element count */
+int qtt_btensor_info(const qtt_btensor* t, int* rank, int* ngroups, int64_t* total_sections, int64_t* nblocks,
+                     int64_t* numel);
+/* Structure readback: nsec[rank], sec_sizes[total_sections],
+ * cvals[total_sections * ngroups], sel[ngroups], group_mod[ngroups]. */
+int qtt_btensor_structure(const qtt_btensor* t, int64_t* nsec, int64_t* sec_sizes, int16_t* cvals, int16_t* sel,
+                          uint16_t* group_mod);
+/* Block readback (synchronising): block_idx[nblocks * rank], data[numel]. */
+int qtt_btensor_blocks(const qtt_btensor* t, int32_t* block_idx, double* data);
+
+/* ---- structural operations -------------------------------------------------- */
+/* btensor::permute (btensor.h:555, sources/btensor.cpp:1754-1802) */
+int qtt_permute(const qtt_btensor* t, const int64_t* perm, qtt_btensor** out);
+/* btensor::reshape (btensor.h:569, sources/btensor.cpp:2986-3024) */
+int qtt_reshape(const qtt_btensor* t, int ngroups, const int64_t* index_groups, qtt_btensor** out);
+/* btensor::conj (btensor.h:649, sources/btensor.cpp:2156) */
+int qtt_conj(const qtt_btensor* t, qtt_btensor** out);
+
+/* ---- contraction ------------------------------------------------------------
+ * btensor::tensordot (include/blockTensor/btensor.h:623) and the free
+ * tensordot(l, r, dl, dr) (:1230-1234), bound in Python at
+ * python_binding/quantt.cpp:581-597. */
+int qtt_tensordot(const qtt_btensor* a, const qtt_btensor* b, int ndims, const int64_t* dims_a,
+                  const int64_t* dims_b, qtt_btensor** out);
+/* Host-counted algorithmic FLOPs (sum of 2 M N K) of the last tensordot on this thread. */
+double qtt_last_tensordot_flops(void);
+
+/* ---- elementwise ------------------------------------------------------------ */
+/* broadcast mul (btensor.h:547, sources/btensor.cpp:1750) */
+int qtt_mul(const qtt_btensor* x, const qtt_btensor* y, qtt_btensor** out);
+/* add(other, alpha) (btensor.h:462, sources/btensor.cpp:2666) */
+int qtt_add(const qtt_btensor* x, const qtt_btensor* y, double alpha, qtt_btensor** out);
+/* rank-0 tensordot(x, y.conj(), all dims) */
+int qtt_full_dot(const qtt_btensor* x, const qtt_btensor* y, double* out);
+
+/* ---- block linear algebra (include/blockTensor/LinearAlgebra.h) --------- */
+/* svd(const btensor&, size_t split) (LinearAlgebra.h:87) */
+int qtt_svd(const qtt_btensor* t, size_t split, qtt_btensor** U, qtt_btensor** d, qtt_btensor** V);
+/* svd(const btensor&, size_t split, tol, min_size, max_size, pow) (LinearAlgebra.h:115-116) */
+int qtt_svd_truncated(const qtt_btensor* t, size_t split, double tol, size_t min_size, size_t max_size, double pow,
+                      qtt_btensor** U, qtt_btensor** d, qtt_btensor** V);
+/* eigh(const btensor&, size_t split) (LinearAlgebra.h:167) */
+int qtt_eigh(const qtt_btensor* t, size_t split, qtt_btensor** d, qtt_btensor** U);
+/* truncate(std::tuple<U, d, V>&&, max, min, tol, pow) (LinearAlgebra.h:244-247) */
+int qtt_truncate(const qtt_btensor* U, const qtt_btensor* d, const qtt_btensor* V, size_t max_size, size_t min_size,
+                 double tol, double pow, qtt_btensor** U_out, qtt_btensor** d_out, qtt_btensor** V_out);
+
+/* ---- DMRG ----------------------------------------------------------------
+ * dmrg(bMPO&, bMPS&, const dmrg_options&, dmrg_logger&) (include/dmrg.h:39-40,
+ * bound at python_binding/algorithm.cpp:46-62).  options mirrors
+ * dmrg_options (include/dmrg_options.h:15-57).  mps[] is replaced in place by
+ * the optimised state (the old handles are destroyed).  Per-sweep energies,
+ * wall times (ms) and largest bond are written for up to max_log sweeps. */
+typedef struct qtt_dmrg_options {
+  double cutoff;
+  double convergence_criterion;
+  uint64_t maximum_bond;
+  uint64_t minimum_bond;
+  uint64_t maximum_iterations;
+} qtt_dmrg_options;
+int qtt_dmrg(int L, qtt_btensor* const* mpo, qtt_btensor** mps, const qtt_dmrg_options* options, double* energy,
+             int max_log, double* sweep_energies, double* sweep_ms, int64_t* max_bond, int* n_sweeps,
+             double* contract_flops, double* svd_flops);
+/* contract(bMPS, bMPS) and contract(bMPS, bMPS, bMPO) (sources/MPT.cpp:211-292) */
+int qtt_contract_norm(int L, qtt_btensor* const* a, qtt_btensor* const* b, double* out);
+int qtt_contract_energy(int L, qtt_btensor* const* a, qtt_btensor* const* b, qtt_btensor* const* mpo, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUANTIT_AMD_H */

@@ -1,0 +1,243 @@
+// Two-site DMRG with one-step Lanczos (sources/dmrg.cpp), every tensor
+// operation on device; the host keeps the sector bookkeeping and reads back
+// only the kept-dimension counts of each truncation and one energy per sweep.
+#include "dmrg.hpp"
+
+#include <chrono>
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace qtt {
+
+// edge_shape_prep (dmrg.cpp:334-342).  The arguments of its final shape_from
+// are evaluated right to left (g++), so the rank-0 part is taken before
+// inverse_cvals_.
+Shape edge_shape_prep(const Shape& t, int dim) {
+  std::vector<int64_t> vec(t.rank(), 0);
+  vec[dim] = -1;
+  Shape S = t.shape_from(vec);
+  S.sel = q_neutral();
+  Shape s0 = S.shape_from({0});
+  Shape Si = S.inverse_cvals();
+  return shape_product({&Si, &s0});
+}
+
+BTensor compute_left_env(const BTensor& W, const BTensor& Y, const BTensor& L) {
+  BTensor out = tensordot(L, Y, {0}, {0});
+  out = tensordot(out, W, {0, 2}, {0, 3});
+  return tensordot(out, conj(Y), {0, 2}, {0, 1});
+}
+
+BTensor compute_right_env(const BTensor& W, const BTensor& Y, const BTensor& R) {
+  BTensor out = tensordot(R, Y, {0}, {2});
+  out = tensordot(out, W, {0, 3}, {2, 3});
+  return tensordot(out, conj(Y), {3, 0}, {1, 2});
+}
+
+BTensor hamil2site_times_state(const BTensor& theta, const BTensor& W12, const BTensor& L, const BTensor& R) {
+  BTensor out = tensordot(L, theta, {0}, {0});
+  out = tensordot(out, W12, {0, 2, 3}, {0, 4, 5});
+  return tensordot(out, R, {1, 4}, {0, 1});
+}
+
+namespace {
+
+BTensor& env_at(DmrgState& st, int i) { return st.env.at(size_t(i + 1)); }
+
+int64_t bond_of(const BTensor& a) {
+  int64_t n = 0;
+  for (auto x : a.shape.secs[2]) n += x;
+  return n;
+}
+
+// rank-0 tensor holding 0 when a Lanczos scalar has no block (no matching
+// block pair): the reference's arithmetic then runs on an empty tensor.
+BTensor zero_scalar(Context* ctx, const Groups& g) {
+  Shape s;
+  s.g = g;
+  s.sel = q_neutral();
+  return fill_like(ctx, s, 0.0);
+}
+
+// eye_like of a rank-2 shape: identity on the diagonal blocks.
+BTensor eye_like(Context* ctx, const Shape& s) {
+  std::vector<BIdx> idx;
+  std::vector<double> host;
+  for (int i = 0; i < s.nsec(0); ++i)
+    for (int j = 0; j < s.nsec(1); ++j) {
+      BIdx b{};
+      b.i[0] = i;
+      b.i[1] = j;
+      if (!s.allowed(b)) continue;
+      const int64_t r = s.secs[0][i], c = s.secs[1][j];
+      if (i != j || r != c) continue;
+      idx.push_back(b);
+      for (int64_t x = 0; x < r; ++x)
+        for (int64_t y = 0; y < c; ++y) host.push_back(x == y ? 1.0 : 0.0);
+    }
+  BTensor out(ctx, s);
+  out.upload(idx, host.data());
+  return out;
+}
+
+}  // namespace
+
+void dmrg_prepare(DmrgState& st) {
+  const int L = int(st.mpo.size());
+  if (L < 2 || int(st.mps.size()) != L) throw InvalidArgument("MPS and MPO must have the same length >= 2");
+  Context* ctx = st.mps[0].ctx;
+  st.oc = 0;
+  st.step = 1;
+  st.env.assign(size_t(L + 2), BTensor());
+  Shape Ls = edge_shape_prep(st.mps.front().shape, 0);
+  Shape LH = edge_shape_prep(st.mpo.front().shape, 0);
+  Shape Lsi = Ls.inverse_cvals();
+  env_at(st, -1) = fill_like(ctx, shape_product({&Ls, &LH, &Lsi}), 1.0);
+  Shape Rs = edge_shape_prep(st.mps.back().shape, 2);
+  Shape RH = edge_shape_prep(st.mpo.back().shape, 2);
+  Shape Rsi = Rs.inverse_cvals();
+  env_at(st, L) = fill_like(ctx, shape_product({&Rs, &RH, &Rsi}), 1.0);
+  for (int i = L - 1; i > st.oc; --i) env_at(st, i) = compute_right_env(st.mpo[i], st.mps[i], env_at(st, i + 1));
+  st.w12.clear();
+  for (int i = 0; i + 1 < L; ++i)
+    st.w12.push_back(permute(tensordot(st.mpo[i], st.mpo[i + 1], {2}, {0}), {0, 1, 3, 4, 2, 5}));
+}
+
+BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
+  const int oc = st.oc;
+  Context* ctx = st.mps[0].ctx;
+  const Groups& g = st.mps[0].shape.g;
+  ContractStats cs;
+  SvdStats ss;
+  BTensor& Lenv = env_at(st, oc - 1);
+  BTensor& Renv = env_at(st, oc + 2);
+  const BTensor& W = st.w12[oc];
+  BTensor theta = tensordot(st.mps[oc], st.mps[oc + 1], {2}, {0}, &cs);
+  // one_step_lanczos (dmrg.cpp:584-608)
+  auto matvec = [&](const BTensor& x) {
+    BTensor out = tensordot(Lenv, x, {0}, {0}, &cs);
+    out = tensordot(out, W, {0, 2, 3}, {0, 4, 5}, &cs);
+    return tensordot(out, Renv, {1, 4}, {0, 1}, &cs);
+  };
+  BTensor psi = matvec(theta);
+  BTensor a0 = full_dot(psi, theta);
+  if (a0.blocks.empty()) a0 = zero_scalar(ctx, g);
+  else psi = add(psi, mul(theta, a0), -1.0);
+  BTensor b = full_dot(psi, psi, true);
+  if (b.blocks.empty()) b = zero_scalar(ctx, g);
+  psi = div_scalar_tensor(psi, b, 1e-15);
+  BTensor a1 = full_dot(matvec(psi), psi);
+  if (a1.blocks.empty()) a1 = zero_scalar(ctx, g);
+  // eig2x2Mat (dmrg.cpp:543-572) on device
+  Shape s0;
+  s0.g = g;
+  s0.sel = q_neutral();
+  BufPtr trio = ctx->alloc(4 * sizeof(double));
+  double* ab = reinterpret_cast<double*>(ctx->arena().base());
+  BTensor E(ctx, s0), o(ctx, s0), n(ctx, s0);
+  for (BTensor* x : {&E, &o, &n}) {
+    x->storage = trio;
+    x->blocks.push_back({BIdx{}, 0, 1});
+  }
+  o.blocks[0].off = 1;
+  n.blocks[0].off = 2;
+  BufPtr flag = ctx->alloc(sizeof(int));
+  kern::fill_i32(ctx->stream(), reinterpret_cast<int32_t*>(flag->cptr()), 1, 0);
+  {
+    KernelTimer kt(*ctx, "eig2x2");
+    kern::eig2x2(ctx->stream(), ab, a0.ptr(a0.blocks[0]) - ab, a1.ptr(a1.blocks[0]) - ab, b.ptr(b.blocks[0]) - ab,
+                 trio->dptr() - ab, reinterpret_cast<int*>(flag->cptr()));
+  }
+  int h_flag = 0;
+  QTT_HIP(hipMemcpyAsync(&h_flag, flag->cptr(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
+  BTensor theta2 = axpby_dev(theta, o, psi, n);
+  SvdResult r = svd_truncated(theta2, 2, opt.cutoff, opt.minimum_bond, opt.maximum_bond, 2.0, &ss);
+  ctx->sync();  // normally already done by the truncation; the flag copy is complete
+  if (h_flag) throw LogicError("nan found in output tensor");
+  normalize_d(r.d);
+  if (st.step == 1) {
+    st.mps[oc] = r.U;
+    st.mps[oc + 1] = permute(conj(mul(r.V, r.d)), {2, 0, 1});
+    env_at(st, oc) = compute_left_env(st.mpo[oc], st.mps[oc], env_at(st, oc - 1));
+  } else {
+    st.mps[oc] = mul(r.U, r.d);
+    st.mps[oc + 1] = permute(conj(r.V), {2, 0, 1});
+    env_at(st, oc + 1) = compute_right_env(st.mpo[oc + 1], st.mps[oc + 1], env_at(st, oc + 2));
+  }
+  if (log) {
+    log->contract_flops += cs.flops;
+    log->svd_flops += ss.flops;
+    log->updates += 1;
+  }
+  return E;
+}
+
+double dmrg_sweep(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
+  const int L = int(st.mpo.size());
+  const int nstep = int(st.w12.size()) - 1 + (st.w12.size() == 1);
+  BTensor E;
+  for (int i = 0; i < 2 * nstep; ++i) {
+    E = dmrg_update(st, opt, log);
+    st.oc += st.step;
+    if (st.oc == 0 || st.oc == L - 2) st.step = -st.step;
+  }
+  return item(E);
+}
+
+double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
+  dmrg_prepare(st);
+  double E0 = 100000.0;
+  for (size_t it = 0; it < opt.maximum_iterations; ++it) {
+    auto t0 = std::chrono::steady_clock::now();
+    const double E = dmrg_sweep(st, opt, log);
+    auto t1 = std::chrono::steady_clock::now();
+    if (log) {
+      log->energies.push_back(E);
+      log->sweep_ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+      int64_t mb = 0;
+      for (auto& a : st.mps) mb = std::max(mb, bond_of(a));
+      log->max_bond.push_back(mb);
+      log->middle_bond.push_back(bond_of(st.mps[st.mps.size() / 2]));
+    }
+    const double Eold = E0;
+    E0 = E;
+    if (!(std::fabs((E0 - Eold) / E0) > opt.convergence_criterion)) break;
+  }
+  return E0;
+}
+
+double contract_norm(const std::vector<BTensor>& a, const std::vector<BTensor>& b) {
+  Context* ctx = a[0].ctx;
+  Shape l1 = a.front().shape.shape_from({-1, 0, 0}).inverse_cvals();
+  Shape l2 = b.front().shape.shape_from({-1, 0, 0});
+  Shape r1 = a.back().shape.shape_from({0, 0, -1}).inverse_cvals();
+  Shape r2 = b.back().shape.shape_from({0, 0, -1});
+  BTensor left = eye_like(ctx, shape_product({&l1, &l2}));
+  BTensor right = eye_like(ctx, shape_product({&r1, &r2}));
+  for (size_t i = 0; i < a.size(); ++i) {
+    left = tensordot(left, a[i], {0}, {0});
+    left = tensordot(left, conj(b[i]), {0, 1}, {0, 1});
+  }
+  return item(tensordot(left, right, {0, 1}, {0, 1}));
+}
+
+double contract_energy(const std::vector<BTensor>& a, const std::vector<BTensor>& b, const std::vector<BTensor>& mpo) {
+  Context* ctx = a[0].ctx;
+  Shape l1 = a.front().shape.shape_from({-1, 0, 0}).inverse_cvals();
+  Shape l2 = b.front().shape.shape_from({-1, 0, 0});
+  Shape r1 = a.back().shape.shape_from({0, 0, -1}).inverse_cvals();
+  Shape r2 = b.back().shape.shape_from({0, 0, -1});
+  BTensor left = eye_like(ctx, shape_product({&l1, &l2}));
+  BTensor right = eye_like(ctx, shape_product({&r1, &r2}));
+  left = permute(tensordot(left, fill_like(ctx, edge_shape_prep(mpo.front().shape, 0), 1.0), {}, {}), {0, 2, 1});
+  right = permute(tensordot(right, fill_like(ctx, edge_shape_prep(mpo.back().shape, 2), 1.0), {}, {}), {0, 2, 1});
+  for (size_t i = 0; i < a.size(); ++i) {
+    left = tensordot(left, a[i], {0}, {0});
+    left = tensordot(left, mpo[i], {0, 2}, {0, 3});
+    left = tensordot(left, conj(b[i]), {0, 2}, {0, 1});
+  }
+  return item(tensordot(left, right, {0, 1, 2}, {0, 1, 2}));
+}
+
+}  // namespace qtt

@@ -1,0 +1,60 @@
+// Two-site DMRG driver on device-resident tensors (sources/dmrg.cpp).
+#pragma once
+
+#include <limits>
+#include <vector>
+
+#include "btensor.hpp"
+
+namespace qtt {
+
+// dmrg_options (include/dmrg_options.h:15-57) with the reference defaults.
+struct DmrgOptions {
+  double cutoff = 1e-6;
+  double convergence_criterion = 1e-5;
+  size_t maximum_bond = std::numeric_limits<size_t>::max();
+  size_t minimum_bond = 4;
+  size_t maximum_iterations = 1000;
+};
+
+// Per-sweep record, the dmrg_logger hooks' content (include/dmrg_logger.h:27-185).
+struct DmrgLog {
+  std::vector<double> energies;   // energy after each sweep
+  std::vector<double> sweep_ms;   // wall time of each sweep
+  std::vector<int64_t> max_bond;  // largest bond after each sweep
+  std::vector<int64_t> middle_bond;
+  double contract_flops = 0, svd_flops = 0;
+  int64_t updates = 0;
+};
+
+struct DmrgState {
+  std::vector<BTensor> mpo;  // W_i (l, s, r, s')
+  std::vector<BTensor> mps;  // A_i (l, s, r), orthogonality centre at 0
+  std::vector<BTensor> w12;  // two-site Hamiltonians
+  std::vector<BTensor> env;  // env[i + 1] = Env[i] for i = -1..L
+  int oc = 0;
+  int step = 1;
+};
+
+// Initialises environments and two-site Hamiltonians (dmrg.cpp:92-100,
+// 370-409, 503-515).
+void dmrg_prepare(DmrgState& st);
+// One call of dmrg_2sites_update::operator() (dmrg.cpp:163-206); returns the
+// rank-0 energy tensor.
+BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
+// One full sweep: 2(L-2) updates (dmrg.cpp:127-142); returns the last energy.
+double dmrg_sweep(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
+// details::dmrg_impl (dmrg.cpp:219-273) from orthogonality centre 0.
+double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
+
+// Environment helpers (dmrg.cpp:334-345, 424-493).
+Shape edge_shape_prep(const Shape& t, int dim);
+BTensor compute_left_env(const BTensor& W, const BTensor& Y, const BTensor& L);
+BTensor compute_right_env(const BTensor& W, const BTensor& Y, const BTensor& R);
+BTensor hamil2site_times_state(const BTensor& theta, const BTensor& W12, const BTensor& L, const BTensor& R);
+
+// <a|b> and <a|H|b> (sources/MPT.cpp:211-292) for checks.
+double contract_norm(const std::vector<BTensor>& a, const std::vector<BTensor>& b);
+double contract_energy(const std::vector<BTensor>& a, const std::vector<BTensor>& b, const std::vector<BTensor>& mpo);
+
+}  // namespace qtt

@@ -1,0 +1,134 @@
+// Host-side launchers of the gfx950 kernels.  Every device pointer is the
+// context arena base plus an element (double) offset, so descriptor lists are
+// plain integer records.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qtt {
+namespace kern {
+
+// ---- grouped block GEMM ---------------------------------------------------
+// C_out = alpha * sum_p A_p * B_p + beta * C_out for every output block, the
+// pairs of one output block taken in list order (the reference's mm +
+// addmm_ order, sources/btensor.cpp:2095-2102).  A_p[m,k] =
+// base[a_off + m*a_rs + k*a_cs], B_p[k,n] = base[b_off + k*b_rs + n*b_cs];
+// C is row-major M x N at c_off.
+struct GemmOut {
+  int64_t c_off;
+  int32_t M, N;
+  int32_t pair_begin, pair_end;
+};
+struct GemmPair {
+  int64_t a_off, b_off;
+  int64_t a_rs, a_cs, b_rs, b_cs;
+  int32_t K, pad;
+};
+struct GemmTile {
+  int32_t out;
+  int16_t tm, tn;  // tile coordinates inside the output block
+};
+constexpr int kGemmTile = 64;
+
+void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
+              const GemmPair* pairs, double alpha, double beta);
+// hi/lo split FP32 MFMA: each operand x = hi + lo (hi = fp32(x), lo =
+// fp32(x - hi)); products hi*hi + hi*lo + lo*hi accumulate in FP32 per K
+// chunk and are folded into FP64 accumulators after every chunk.
+void gemm_split(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
+                const GemmPair* pairs, double alpha, double beta);
+
+// ---- data movement ----------------------------------------------------------
+// Row-major copy of a permuted view: dst element with multi-index j over
+// dims[0..rank) (row-major) reads src[sum_k j_k * sstride[k]].
+struct PermDesc {
+  int64_t src_off, dst_off, numel;
+  int32_t rank, pad;
+  int32_t dims[8];
+  int64_t sstride[8];
+};
+struct ChunkTask {
+  int32_t desc;
+  int32_t chunk;
+};
+constexpr int kChunk = 8192;
+void permute_copy(hipStream_t s, double* base, const PermDesc* descs, const ChunkTask* tasks, int ntasks);
+
+// 2-D strided copy dst[r*dst_rs + c*dst_cs] = alpha * src[r*src_rs + c*src_cs];
+// src_off < 0 fills the destination with the value alpha.
+struct Copy2D {
+  int64_t src_off, dst_off;
+  int64_t src_rs, src_cs, dst_rs, dst_cs;
+  int32_t rows, cols;
+  double alpha;
+};
+void copy2d(hipStream_t s, double* base, const Copy2D* descs, const ChunkTask* tasks, int ntasks);
+
+// ---- elementwise / reductions ---------------------------------------------
+// out = x (op) y with y broadcast: mode 0 y scalar, 1 y vector along the last
+// dim (x viewed as rows x ncol), 2 y same shape.  op 0 = mul, 1 = div.
+// For op 1 with mode 0 and min_abs > 0 the division is skipped (plain copy)
+// when |y| < min_abs.
+struct EwDesc {
+  int64_t x_off, y_off, out_off, numel;
+  int32_t ncol, mode;
+};
+void ew_binary(hipStream_t s, double* base, const EwDesc* descs, const ChunkTask* tasks, int ntasks, int op,
+               double min_abs);
+// out = a*x + b*y; x_off or y_off < 0 means that operand is absent.  a and b
+// are read from device scalars when a_off/b_off >= 0, else the host values.
+struct AxpbyDesc {
+  int64_t x_off, y_off, out_off, numel;
+};
+void axpby(hipStream_t s, double* base, const AxpbyDesc* descs, const ChunkTask* tasks, int ntasks, int64_t a_off,
+           double a, int64_t b_off, double b);
+// Deterministic dot product over matched block pairs: one partial per chunk
+// task, then a fixed-order tree reduction into base[out_off].  With sqrt_out
+// the square root is stored instead.
+struct DotDesc {
+  int64_t x_off, y_off, numel;
+};
+void dot(hipStream_t s, double* base, const DotDesc* descs, const ChunkTask* tasks, int ntasks,
+         int64_t partial_off, int64_t out_off, bool sqrt_out);
+// eig2x2Mat (sources/dmrg.cpp:543-572) on device scalars a0, a1, b; writes
+// E, o, n to out[0..3) and sets flags[0] = 1 on NaN.
+void eig2x2(hipStream_t s, double* base, int64_t a0_off, int64_t a1_off, int64_t b_off, int64_t out_off,
+            int* flags);
+// d /= sqrt(sum d^2) over n contiguous values.
+void normalize(hipStream_t s, double* base, int64_t off, int64_t n);
+
+// ---- block SVD ----------------------------------------------------------
+// Sector work matrices: Y (ell x k, column-major, ld = ell) is orthogonalised
+// in place by block one-sided Jacobi; W (k x k, column-major) accumulates the
+// rotations.  See svd_jacobi.hip.
+struct SvdSector {
+  int64_t y_off, w_off;     // work matrices
+  int64_t u_off, v_off;     // outputs: U (m x k) and V (n x k) row-major
+  int64_t s_off;            // singular values (k), descending
+  int32_t m, n, ell, k;     // ell = max(m,n), k = min(m,n)
+  int32_t nb;               // number of column blocks (even)
+  int32_t transposed;       // 1 when Y = A^T (m < n)
+  int32_t flag_idx;         // rotation counter slot
+  int32_t eig_shift;        // 1: eigh via shifted SVD (A + c I), values written ascending
+  int64_t shift_off;        // device double holding c (eigh mode)
+};
+constexpr int kSvdBlock = 16;  // columns per Jacobi block
+void svd_init_w(hipStream_t s, double* base, const SvdSector* secs, int nsec);
+void svd_round(hipStream_t s, double* base, const SvdSector* secs, const int32_t* work, int nwork, int round,
+               int* counters, double tol);
+void svd_finalize(hipStream_t s, double* base, const SvdSector* secs, int nsec);
+// eigh mode: c = ||A||_F per sector, stored at shift_off, and Y += c I.
+void svd_shift(hipStream_t s, double* base, const SvdSector* secs, int nsec);
+void fill_i32(hipStream_t s, int32_t* p, int n, int32_t v);
+
+// Global truncation (sources/btensor_linalg.cpp:657-755,
+// sources/LinearAlgebra.cpp:57-75): singular values of nsec sectors stored
+// descending at s_off[i] with k[i] entries; writes kept[i].
+void truncate_select(hipStream_t s, double* base, const int64_t* s_offs, const int32_t* ks, int nsec,
+                     int64_t scratch_off, int64_t total, double tol, double pow, int64_t min_size,
+                     int64_t max_size, int32_t* kept, double* thr_out);
+
+}  // namespace kern
+}  // namespace qtt

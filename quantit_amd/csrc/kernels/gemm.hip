@@ -1,0 +1,233 @@
+// Grouped block-sparse GEMM for gfx950 (CDNA4).
+//
+// One work-group computes one 64x64 tile of one output block and walks that
+// block's list of (A_p, B_p) pairs in order, so every output block is written
+// exactly once with a fixed accumulation order (no atomics: results do not
+// depend on what else runs on the device).  The permute gather of the
+// reference's permute_bl (sources/btensor.cpp:1843-1894) is folded into the
+// tile loads through per-pair (row, column) element strides.
+//
+// FP64 path: v_mfma_f64_16x16x4_f64.  Each of the 4 waves owns a 32x32
+// quadrant = 2x2 MFMA tiles; LDS holds a 64x16 A tile and a 16x64 B tile
+// (k-major, row pitch 80 doubles so the two 16-lane halves of a 32-lane
+// LDS read phase fall in opposite bank halves).
+#include "../kernels.hpp"
+
+namespace qtt {
+namespace kern {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int LDP = BM + 16;  // LDS row pitch (doubles)
+
+// Loads the A tile (BM x BK, element (m,k) at A[m*rs + k*cs]) into As[k][m]
+// and the B tile (BK x BN, element (k,n) at B[k*rs + n*cs]) into Bs[k][n].
+template <class T>
+__device__ __forceinline__ void load_tiles(const double* __restrict__ A, const double* __restrict__ B,
+                                           const GemmPair& pr, int M, int N, int m0, int n0, int k0, int tid,
+                                           T (*As)[LDP], T (*Bs)[LDP], T (*As2)[LDP], T (*Bs2)[LDP]) {
+  const int K = pr.K;
+  if (pr.a_cs == 1) {  // row-major A: 4 consecutive k per thread
+    const int m = tid >> 2, kq = (tid & 3) * 4;
+    const int gm = m0 + m;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gk = k0 + kq + j;
+      double v = (gm < M && gk < K) ? A[int64_t(gm) * pr.a_rs + gk] : 0.0;
+      if constexpr (sizeof(T) == 8) {
+        As[kq + j][m] = v;
+      } else {
+        float hi = float(v);
+        As[kq + j][m] = hi;
+        As2[kq + j][m] = float(v - double(hi));
+      }
+    }
+  } else {  // column-major or general strides: 4 consecutive m per thread
+    const int k = tid >> 4, mq = (tid & 15) * 4;
+    const int gk = k0 + k;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gm = m0 + mq + j;
+      double v = (gm < M && gk < K) ? A[int64_t(gm) * pr.a_rs + int64_t(gk) * pr.a_cs] : 0.0;
+      if constexpr (sizeof(T) == 8) {
+        As[k][mq + j] = v;
+      } else {
+        float hi = float(v);
+        As[k][mq + j] = hi;
+        As2[k][mq + j] = float(v - double(hi));
+      }
+    }
+  }
+  if (pr.b_cs == 1) {  // row-major B: 4 consecutive n per thread
+    const int k = tid >> 4, nq = (tid & 15) * 4;
+    const int gk = k0 + k;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gn = n0 + nq + j;
+      double v = (gn < N && gk < K) ? B[int64_t(gk) * pr.b_rs + gn] : 0.0;
+      if constexpr (sizeof(T) == 8) {
+        Bs[k][nq + j] = v;
+      } else {
+        float hi = float(v);
+        Bs[k][nq + j] = hi;
+        Bs2[k][nq + j] = float(v - double(hi));
+      }
+    }
+  } else {  // column-major or general: 4 consecutive k per thread
+    const int n = tid >> 2, kq = (tid & 3) * 4;
+    const int gn = n0 + n;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gk = k0 + kq + j;
+      double v = (gn < N && gk < K) ? B[int64_t(gk) * pr.b_rs + int64_t(gn) * pr.b_cs] : 0.0;
+      if constexpr (sizeof(T) == 8) {
+        Bs[kq + j][n] = v;
+      } else {
+        float hi = float(v);
+        Bs[kq + j][n] = hi;
+        Bs2[kq + j][n] = float(v - double(hi));
+      }
+    }
+  }
+}
+
+// Output element (row, col) of 16x16 MFMA tile register r in lane l:
+//   v_mfma_f64_16x16x4_f64: row = 4*r + l/16, col = l%16
+//   v_mfma_f32_16x16x4_f32: row = 4*(l/16) + r, col = l%16
+// (measured on gfx950; the two layouts differ).
+template <bool F64Layout>
+__device__ __forceinline__ void store_tile(double* __restrict__ C, const GemmOut& o, int m0, int n0, int wm, int wn,
+                                           int lane, const v4d (&acc)[2][2], double alpha, double beta) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + (F64Layout ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r);
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        if (row < o.M && col < o.N) {
+          double* p = C + int64_t(row) * o.N + col;
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * (*p);
+          *p = v;
+        }
+      }
+}
+
+__global__ __launch_bounds__(256) void k_gemm_f64(double* __restrict__ base, const GemmTile* __restrict__ tiles,
+                                                  const GemmOut* __restrict__ outs,
+                                                  const GemmPair* __restrict__ pairs, double alpha, double beta) {
+  __shared__ double As[BK][LDP];
+  __shared__ double Bs[BK][LDP];
+  const GemmTile t = tiles[blockIdx.x];
+  const GemmOut o = outs[t.out];
+  const int m0 = t.tm * BM, n0 = t.tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  for (int p = o.pair_begin; p < o.pair_end; ++p) {
+    const GemmPair pr = pairs[p];
+    const double* A = base + pr.a_off;
+    const double* B = base + pr.b_off;
+    for (int k0 = 0; k0 < pr.K; k0 += BK) {
+      load_tiles<double>(A, B, pr, o.M, o.N, m0, n0, k0, tid, As, Bs, nullptr, nullptr);
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4), c = lane & 15;
+        const double a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
+        const double b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+  store_tile<true>(base + o.c_off, o, m0, n0, wm, wn, lane, acc, alpha, beta);
+}
+
+__global__ __launch_bounds__(256) void k_gemm_split(double* __restrict__ base, const GemmTile* __restrict__ tiles,
+                                                    const GemmOut* __restrict__ outs,
+                                                    const GemmPair* __restrict__ pairs, double alpha,
+                                                    double beta) {
+  __shared__ float Ah[BK][LDP], Al[BK][LDP];
+  __shared__ float Bh[BK][LDP], Bl[BK][LDP];
+  const GemmTile t = tiles[blockIdx.x];
+  const GemmOut o = outs[t.out];
+  const int m0 = t.tm * BM, n0 = t.tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  for (int p = o.pair_begin; p < o.pair_end; ++p) {
+    const GemmPair pr = pairs[p];
+    const double* A = base + pr.a_off;
+    const double* B = base + pr.b_off;
+    for (int k0 = 0; k0 < pr.K; k0 += BK) {
+      load_tiles<float>(A, B, pr, o.M, o.N, m0, n0, k0, tid, Ah, Bh, Al, Bl);
+      __syncthreads();
+      v4f hh[2][2], cr[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          hh[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+          cr[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4), c = lane & 15;
+        float ah[2] = {Ah[kr][wm + c], Ah[kr][wm + 16 + c]};
+        float al[2] = {Al[kr][wm + c], Al[kr][wm + 16 + c]};
+        float bh[2] = {Bh[kr][wn + c], Bh[kr][wn + 16 + c]};
+        float bl[2] = {Bl[kr][wn + c], Bl[kr][wn + 16 + c]};
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            hh[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[i], bh[j], hh[i][j], 0, 0, 0);
+            cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[i], bl[j], cr[i][j], 0, 0, 0);
+            cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[i], bh[j], cr[i][j], 0, 0, 0);
+          }
+      }
+      // fold this K chunk into the FP64 accumulators
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[i][j][r] += double(hh[i][j][r]) + double(cr[i][j][r]);
+      __syncthreads();
+    }
+  }
+  store_tile<false>(base + o.c_off, o, m0, n0, wm, wn, lane, acc, alpha, beta);
+}
+
+void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
+              const GemmPair* pairs, double alpha, double beta) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_gemm_f64, dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
+}
+
+void gemm_split(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
+                const GemmPair* pairs, double alpha, double beta) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_gemm_split, dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
+}
+
+}  // namespace kern
+}  // namespace qtt

@@ -1,0 +1,517 @@
+// Block linear algebra: sector grouping of btensor_linalg.cpp on the host,
+// batched Jacobi SVD / eigh and the global truncation on device.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <limits>
+#include <numeric>
+
+#include "btensor.hpp"
+#include "kernels.hpp"
+
+namespace qtt {
+
+namespace {
+
+double* arena_base(Context* c) { return reinterpret_cast<double*>(c->arena().base()); }
+
+void add_chunks(std::vector<kern::ChunkTask>& tasks, int desc, int64_t numel) {
+  const int64_t nch = (numel + kern::kChunk - 1) / kern::kChunk;
+  for (int64_t c = 0; c < nch; ++c) tasks.push_back({desc, int32_t(c)});
+}
+
+struct Slice {
+  int sec;
+  int64_t start, size;
+};
+struct SectorPlan {
+  std::vector<int> blocks;  // indices into the rank-2 tensor's block list
+  std::vector<Slice> rows;  // encounter order
+  std::vector<Slice> cols;  // sorted by section index, offsets in encounter order
+  int64_t m = 0, n = 0;
+};
+
+// reorder_by_cvals + compact_dense + compact_dense_single
+// (sources/btensor_linalg.cpp:30-255) for a rank-2 tensor.
+std::vector<SectorPlan> compact_dense_plan(const BTensor& t) {
+  const Shape& s = t.shape;
+  std::vector<int> ord(t.blocks.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  auto rc = [&](int k) { return s.cvals[0][t.blocks[k].idx.i[0]]; };
+  auto cc = [&](int k) { return s.cvals[1][t.blocks[k].idx.i[1]]; };
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    const Charge ra = rc(a), rb = rc(b);
+    if (ra < rb) return true;
+    return ra == rb && cc(a) < cc(b);
+  });
+  std::vector<SectorPlan> out;
+  size_t st = 0;
+  while (st < ord.size()) {
+    size_t en = st + 1;
+    while (en < ord.size() && rc(ord[en]) == rc(ord[st]) && cc(ord[en]) == cc(ord[st])) ++en;
+    SectorPlan p;
+    int cur_row = t.blocks[ord[st]].idx.i[0];
+    p.rows.push_back({cur_row, 0, s.secs[0][cur_row]});
+    p.m = s.secs[0][cur_row];
+    for (size_t q = st; q < en; ++q) {
+      const BIdx& bi = t.blocks[ord[q]].idx;
+      p.blocks.push_back(ord[q]);
+      const int c = bi.i[1];
+      auto it = std::lower_bound(p.cols.begin(), p.cols.end(), c, [](const Slice& x, int v) { return x.sec < v; });
+      if (it == p.cols.end() || it->sec != c) {
+        p.cols.insert(it, Slice{c, p.n, s.secs[1][c]});
+        p.n += s.secs[1][c];
+      }
+      if (bi.i[0] != cur_row) {
+        cur_row = bi.i[0];
+        p.rows.push_back({cur_row, p.m, s.secs[0][cur_row]});
+        p.m += s.secs[0][cur_row];
+      }
+    }
+    out.push_back(std::move(p));
+    st = en;
+  }
+  return out;
+}
+
+struct Batched {
+  std::vector<kern::SvdSector> secs;
+  BufPtr work, outbuf;
+  std::vector<int64_t> u_off, v_off, s_off;  // element offsets into outbuf
+  int sweeps = 0;
+};
+
+// Gathers the sector matrices of a rank-2 tensor and runs the batched Jacobi
+// SVD (eigh_mode: shifted symmetric eigen-decomposition) on device.
+Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool eigh_mode) {
+  Context* ctx = t.ctx;
+  double* ab = arena_base(ctx);
+  Batched B;
+  const int ns = int(plans.size());
+  B.secs.resize(ns);
+  int64_t work_elems = 0, out_elems = 0;
+  std::vector<int64_t> yo(ns), wo(ns);
+  B.u_off.resize(ns);
+  B.v_off.resize(ns);
+  B.s_off.resize(ns);
+  for (int i = 0; i < ns; ++i) {
+    const auto& p = plans[i];
+    if (eigh_mode && p.m != p.n) throw InvalidArgument("eigh requires square sectors");
+    auto& s = B.secs[i];
+    s.m = int32_t(p.m);
+    s.n = int32_t(p.n);
+    s.transposed = p.m < p.n;
+    s.ell = int32_t(std::max(p.m, p.n));
+    s.k = int32_t(std::min(p.m, p.n));
+    const int nblk = std::max(1, (s.k + kern::kSvdBlock - 1) / kern::kSvdBlock);
+    s.nb = 2 * ((nblk + 1) / 2);
+    const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+    s.flag_idx = i;
+    s.eig_shift = eigh_mode;
+    yo[i] = work_elems;
+    work_elems += int64_t(s.ell) * kp;
+    wo[i] = work_elems;
+    work_elems += kp * kp + 2 * kp + s.ell + 1;  // W + finalize scratch + eigh shift
+    B.u_off[i] = out_elems;
+    out_elems += int64_t(s.m) * s.k;
+    B.v_off[i] = out_elems;
+    out_elems += eigh_mode ? 0 : int64_t(s.n) * s.k;
+    B.s_off[i] = out_elems;
+    out_elems += s.k;
+  }
+  B.work = ctx->alloc(size_t(std::max<int64_t>(work_elems, 1)) * sizeof(double));
+  B.outbuf = ctx->alloc(size_t(std::max<int64_t>(out_elems, 1)) * sizeof(double));
+  const int64_t wb = B.work->dptr() - ab, ob = B.outbuf->dptr() - ab;
+  for (int i = 0; i < ns; ++i) {
+    auto& s = B.secs[i];
+    const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+    s.y_off = wb + yo[i];
+    s.w_off = wb + wo[i];
+    s.shift_off = s.w_off + kp * kp + 2 * kp + s.ell;
+    s.u_off = ob + B.u_off[i];
+    s.v_off = ob + B.v_off[i];
+    s.s_off = ob + B.s_off[i];
+  }
+  // gather: zero the Y matrices, then copy every block into its sector slot
+  std::vector<kern::Copy2D> cp;
+  std::vector<kern::ChunkTask> tasks;
+  for (int i = 0; i < ns; ++i) {
+    const auto& s = B.secs[i];
+    const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+    kern::Copy2D z{};
+    z.src_off = -1;
+    z.dst_off = s.y_off;
+    z.rows = 1;
+    z.cols = int32_t(int64_t(s.ell) * kp);
+    z.dst_rs = 0;
+    z.dst_cs = 1;
+    add_chunks(tasks, int(cp.size()), int64_t(z.cols));
+    cp.push_back(z);
+  }
+  if (!tasks.empty()) {
+    KernelTimer kt(*ctx, "svd_zero");
+    kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
+  }
+  cp.clear();
+  tasks.clear();
+  for (int i = 0; i < ns; ++i) {
+    const auto& p = plans[i];
+    const auto& s = B.secs[i];
+    for (int bk : p.blocks) {
+      const Block& b = t.blocks[bk];
+      if (!b.numel) continue;
+      const auto rit = std::find_if(p.rows.begin(), p.rows.end(), [&](const Slice& x) { return x.sec == b.idx.i[0]; });
+      const auto cit = std::find_if(p.cols.begin(), p.cols.end(), [&](const Slice& x) { return x.sec == b.idx.i[1]; });
+      kern::Copy2D c{};
+      c.src_off = t.ptr(b) - ab;
+      c.rows = int32_t(rit->size);
+      c.cols = int32_t(cit->size);
+      c.src_rs = cit->size;
+      c.src_cs = 1;
+      c.alpha = 1.0;
+      if (!s.transposed) {  // Y = A (col-major, ld = m)
+        c.dst_off = s.y_off + rit->start + cit->start * int64_t(s.ell);
+        c.dst_rs = 1;
+        c.dst_cs = s.ell;
+      } else {  // Y = A^T (col-major, ld = n)
+        c.dst_off = s.y_off + cit->start + rit->start * int64_t(s.ell);
+        c.dst_rs = s.ell;
+        c.dst_cs = 1;
+      }
+      add_chunks(tasks, int(cp.size()), b.numel);
+      cp.push_back(c);
+    }
+  }
+  if (!tasks.empty()) {
+    KernelTimer kt(*ctx, "svd_gather");
+    kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
+  }
+  if (ns == 0) return B;
+  auto* d_secs = ctx->upload_vec(B.secs);
+  {
+    KernelTimer kt(*ctx, "svd_init");
+    kern::svd_init_w(ctx->stream(), ab, d_secs, ns);
+    if (eigh_mode) kern::svd_shift(ctx->stream(), ab, d_secs, ns);
+  }
+  BufPtr cnt = ctx->alloc(size_t(ns) * sizeof(int32_t));
+  int32_t* d_cnt = reinterpret_cast<int32_t*>(cnt->cptr());
+  std::vector<char> active(ns, 1);
+  std::vector<int32_t> h_cnt(ns);
+  int max_ell = 0;
+  for (auto& s : B.secs) max_ell = std::max(max_ell, s.ell);
+  const double tol = 8.0 * std::sqrt(double(std::max(max_ell, 16))) * DBL_EPSILON;
+  const int max_sweeps = 20;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    std::vector<int32_t> work;
+    int max_nb = 0;
+    for (int i = 0; i < ns; ++i) {
+      if (!active[i]) continue;
+      const int nb = B.secs[i].nb;
+      max_nb = std::max(max_nb, nb);
+      for (int pi = 0; pi < nb / 2; ++pi) {
+        work.push_back(i);
+        work.push_back(pi);
+      }
+    }
+    if (work.empty()) break;
+    kern::fill_i32(ctx->stream(), d_cnt, ns, 0);
+    auto* d_work = ctx->upload_vec(work);
+    for (int r = 0; r < max_nb - 1; ++r) {
+      KernelTimer kt(*ctx, "svd_round");
+      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol);
+    }
+    QTT_HIP(hipMemcpyAsync(h_cnt.data(), d_cnt, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx->stream()));
+    ctx->sync();
+    B.sweeps = sweep + 1;
+    bool any = false;
+    for (int i = 0; i < ns; ++i) {
+      if (active[i] && h_cnt[i] == 0) active[i] = 0;
+      any |= bool(active[i]);
+    }
+    if (!any) break;
+  }
+  {
+    KernelTimer kt(*ctx, "svd_finalize");
+    kern::svd_finalize(ctx->stream(), ab, d_secs, ns);
+  }
+  return B;
+}
+
+}  // namespace
+
+// svd(const btensor&, some, uv) on the rank-2 reshaped tensor + reshape_as
+// back (sources/btensor_linalg.cpp:390-534).
+SvdResult svd(const BTensor& t, size_t split, SvdStats* stats) {
+  Context* ctx = t.ctx;
+  if (split > size_t(t.rank())) throw InvalidArgument("split larger than the rank");
+  BTensor rt = reshape(t, {int64_t(split)});
+  const Shape& rs = rt.shape;
+  const Groups& g = rs.g;
+  auto plans = compact_dense_plan(rt);
+  Batched B = run_batched(rt, plans, false);
+  const int ns = int(plans.size());
+  // d (neutral), U (rows, k) with the row charges and rule = rule of t,
+  // V (cols, k) with inverse charges and neutral rule
+  std::vector<int64_t> dsz(ns);
+  std::vector<Charge> rd_c(ns), ld_c(ns);
+  for (int i = 0; i < ns; ++i) {
+    dsz[i] = B.secs[i].k;
+    rd_c[i] = rs.cvals[1][plans[i].cols[0].sec];
+    ld_c[i] = q_inv(g, rd_c[i]);
+  }
+  Shape ds;
+  ds.g = g;
+  ds.secs = {dsz};
+  ds.cvals = {std::vector<Charge>(ns, q_neutral())};
+  ds.sel = q_neutral();
+  Shape us;
+  us.g = g;
+  us.secs = {rs.secs[0], dsz};
+  us.cvals = {rs.cvals[0], rd_c};
+  us.sel = rs.sel;
+  Shape vs;
+  vs.g = g;
+  vs.secs = {rs.secs[1], dsz};
+  vs.cvals = {rs.cvals[1], ld_c};
+  vs.sel = q_neutral();
+  vs = vs.inverse_cvals();
+  BTensor d(ctx, ds), U(ctx, us), V(ctx, vs);
+  d.storage = U.storage = V.storage = B.outbuf;
+  for (int i = 0; i < ns; ++i) {
+    const int64_t k = B.secs[i].k;
+    BIdx di{};
+    di.i[0] = i;
+    d.blocks.push_back({di, B.s_off[i], k});
+    for (auto& r : plans[i].rows) {
+      BIdx ui{};
+      ui.i[0] = r.sec;
+      ui.i[1] = i;
+      U.blocks.push_back({ui, B.u_off[i] + r.start * k, r.size * k});
+    }
+    for (auto& c : plans[i].cols) {
+      BIdx vi{};
+      vi.i[0] = c.sec;
+      vi.i[1] = i;
+      V.blocks.push_back({vi, B.v_off[i] + c.start * k, c.size * k});
+    }
+  }
+  auto srt = [](BTensor& x) {
+    const int r = x.rank();
+    std::sort(x.blocks.begin(), x.blocks.end(),
+              [r](const Block& a, const Block& b) { return bidx_less(a.idx, b.idx, r); });
+  };
+  srt(U);
+  srt(V);
+  if (stats) {
+    for (auto& s : B.secs) {
+      const double m = std::max(s.m, s.n), n = std::min(s.m, s.n);
+      stats->flops += 14.0 * m * n * n + 8.0 * n * n * n;
+      stats->max_k = std::max(stats->max_k, s.k);
+    }
+    stats->sectors += ns;
+    stats->sweeps = std::max(stats->sweeps, B.sweeps);
+  }
+  // reshape_as back to the original legs
+  std::vector<int64_t> ushape(t.rank(), -1), vshape(t.rank(), -1);
+  for (int i = 0; i < t.rank(); ++i) (i < int(split) ? vshape : ushape)[i] = 0;
+  Shape u_left = t.shape.shape_from(ushape);
+  Shape u_right = U.shape.shape_from({0, -1});
+  Shape v_left = t.shape.shape_from(vshape).inverse_cvals();
+  Shape v_right = V.shape.shape_from({0, -1});
+  SvdResult out;
+  out.U = reshape_as(U, shape_product({&u_left, &u_right}));
+  out.V = reshape_as(V, shape_product({&v_left, &v_right}));
+  out.d = d;
+  return out;
+}
+
+// eigh(const btensor&, split) (sources/btensor_linalg.cpp:294-389): shifted
+// symmetric Jacobi, eigenvalues ascending per sector.
+EighResult eigh(const BTensor& t, size_t split) {
+  Context* ctx = t.ctx;
+  BTensor rt = reshape(t, {int64_t(split)});
+  const Shape& rs = rt.shape;
+  const Groups& g = rs.g;
+  auto plans = compact_dense_plan(rt);
+  Batched B = run_batched(rt, plans, true);
+  const int ns = int(plans.size());
+  std::vector<int64_t> dsz(ns);
+  std::vector<Charge> rd_c(ns);
+  for (int i = 0; i < ns; ++i) {
+    dsz[i] = B.secs[i].k;
+    rd_c[i] = rs.cvals[1][plans[i].cols[0].sec];
+  }
+  Shape ds;
+  ds.g = g;
+  ds.secs = {dsz};
+  ds.cvals = {std::vector<Charge>(ns, q_neutral())};
+  ds.sel = q_neutral();
+  Shape us;
+  us.g = g;
+  us.secs = {rs.secs[0], dsz};
+  us.cvals = {rs.cvals[0], rd_c};
+  us.sel = q_add(g, rs.shape_from({-1, 0}).sel, q_inv(g, rs.cvals[1].at(0)));
+  BTensor d(ctx, ds), U(ctx, us);
+  d.storage = U.storage = B.outbuf;
+  for (int i = 0; i < ns; ++i) {
+    const int64_t k = B.secs[i].k;
+    BIdx di{};
+    di.i[0] = i;
+    d.blocks.push_back({di, B.s_off[i], k});
+    for (auto& r : plans[i].rows) {
+      BIdx ui{};
+      ui.i[0] = r.sec;
+      ui.i[1] = i;
+      if (!us.allowed(ui))
+        throw InvalidArgument("block index " + fmt_idx(ui, 2) + " not allowed by selection rule");
+      U.blocks.push_back({ui, B.u_off[i] + r.start * k, r.size * k});
+    }
+  }
+  const int r = 2;
+  std::sort(U.blocks.begin(), U.blocks.end(), [r](const Block& a, const Block& b) { return bidx_less(a.idx, b.idx, r); });
+  std::vector<int64_t> ushape(t.rank(), -1);
+  for (int i = int(split); i < t.rank(); ++i) ushape[i] = 0;
+  Shape u_left = t.shape.shape_from(ushape);
+  Shape u_right = U.shape.shape_from({0, -1});
+  EighResult out;
+  out.U = reshape_as(U, shape_product({&u_left, &u_right}));
+  out.d = d;
+  return out;
+}
+
+namespace {
+
+// remove_unit_blocks lambda of truncate_impl (sources/btensor_linalg.cpp:690-709),
+// step for step: it advances past one matching block per iteration, so a
+// matching block that directly follows another matching block survives.
+void remove_unit_blocks(std::vector<Block>& lst, int rank, int kd) {
+  size_t src = 0, dest = 0;
+  const size_t n = lst.size();
+  while (dest != n) {
+    const int dest_last = lst[dest].idx.i[rank - 1];
+    dest += (dest_last == kd);
+    if (dest != src && dest != n) std::swap(lst[dest], lst[src]);
+    dest += (dest != n);
+    ++src;
+  }
+  lst.resize(src);
+}
+
+}  // namespace
+
+// truncate_impl (sources/btensor_linalg.cpp:657-755) with the cut computed on
+// device; U and V keep their block order, sliced blocks are compacted into
+// fresh storage.
+SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol, double pw) {
+  BTensor& d = in.d;
+  Context* ctx = d.ctx;
+  double* ab = arena_base(ctx);
+  const int nd = int(d.blocks.size());
+  if (nd == 0) return std::move(in);
+  std::vector<int64_t> s_offs(nd);
+  std::vector<int32_t> ks(nd);
+  int64_t total = 0;
+  for (int i = 0; i < nd; ++i) {
+    s_offs[i] = d.ptr(d.blocks[i]) - ab;
+    ks[i] = int32_t(d.blocks[i].numel);
+    total += ks[i];
+  }
+  if (total == 0) return std::move(in);
+  int64_t n2 = 1;
+  while (n2 < total) n2 <<= 1;
+  BufPtr scratch = ctx->alloc(size_t(n2 + 1) * sizeof(double) + size_t(nd) * sizeof(int32_t) + 256);
+  int32_t* d_kept = reinterpret_cast<int32_t*>(scratch->cptr() + size_t(n2 + 1) * sizeof(double));
+  double* d_thr = scratch->dptr() + n2;
+  const int64_t clamp = std::numeric_limits<int64_t>::max();
+  {
+    KernelTimer kt(*ctx, "truncate_select");
+    kern::truncate_select(ctx->stream(), ab, ctx->upload_vec(s_offs), ctx->upload_vec(ks), nd,
+                          scratch->dptr() - ab, total, tol, pw, int64_t(std::min<size_t>(min_size, size_t(clamp))),
+                          int64_t(std::min<size_t>(max_size, size_t(clamp))), d_kept, d_thr);
+  }
+  std::vector<int32_t> kept(nd);
+  QTT_HIP(hipMemcpyAsync(kept.data(), d_kept, size_t(nd) * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream()));
+  ctx->sync();
+
+  BTensor& U = in.U;
+  BTensor& V = in.V;
+  const int ru = U.rank(), rv = V.rank();
+  std::vector<Block> ub = U.blocks, vb = V.blocks, db = d.blocks;
+  std::vector<int> slice_to(nd, -1);  // per d section: new column count, -1 = unchanged
+  Shape dsh = d.shape, ush = U.shape, vsh = V.shape;
+  std::vector<char> d_erased(nd, 0);
+  for (int i = nd - 1; i >= 0; --i) {
+    const int kd = db[i].idx.i[0];
+    if (kept[i] == 0) {
+      remove_unit_blocks(ub, ru, kd);
+      remove_unit_blocks(vb, rv, kd);
+      d_erased[i] = 1;
+    } else {
+      dsh.secs[0][kd] = kept[i];
+      ush.secs[ru - 1][kd] = kept[i];
+      vsh.secs[rv - 1][kd] = kept[i];
+      slice_to[kd] = kept[i];
+    }
+  }
+  auto rebuild = [&](const BTensor& src, const std::vector<Block>& keep, const Shape& nsh, int r) {
+    BTensor out(ctx, nsh);
+    std::vector<BIdx> idx;
+    for (auto& b : keep) idx.push_back(b.idx);
+    // keep the (possibly quirk-reordered) order of the reference's list
+    out.blocks.clear();
+    int64_t off = 0;
+    for (auto& i : idx) {
+      const int64_t n = nsh.block_numel(i);
+      out.blocks.push_back({i, off, n});
+      off += n;
+    }
+    out.storage = ctx->alloc(size_t(std::max<int64_t>(off, 1)) * sizeof(double));
+    std::vector<kern::Copy2D> cp;
+    std::vector<kern::ChunkTask> tasks;
+    for (size_t q = 0; q < keep.size(); ++q) {
+      const Block& sb = keep[q];
+      const Block& nbk = out.blocks[q];
+      if (!nbk.numel) continue;
+      int64_t dims[kMaxRank];
+      src.shape.block_dims(sb.idx, dims);
+      const int64_t oldk = dims[r - 1];
+      const int64_t rows = oldk ? sb.numel / oldk : 0;
+      const int64_t newk = nsh.secs[r - 1][sb.idx.i[r - 1]];
+      kern::Copy2D c{};
+      c.src_off = src.ptr(sb) - ab;
+      c.dst_off = out.ptr(nbk) - ab;
+      c.rows = int32_t(rows);
+      c.cols = int32_t(newk);
+      c.src_rs = oldk;
+      c.src_cs = 1;
+      c.dst_rs = newk;
+      c.dst_cs = 1;
+      c.alpha = 1.0;
+      add_chunks(tasks, int(cp.size()), rows * newk);
+      cp.push_back(c);
+    }
+    if (!tasks.empty()) {
+      KernelTimer kt(*ctx, "truncate_copy");
+      kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
+    }
+    return out;
+  };
+  // a surviving unitary block of an erased sector keeps its stale size: the
+  // section size of an erased sector is never changed
+  std::vector<Block> dkeep;
+  for (int i = 0; i < nd; ++i)
+    if (!d_erased[i]) dkeep.push_back(db[i]);
+  SvdResult out;
+  out.U = rebuild(U, ub, ush, ru);
+  out.V = rebuild(V, vb, vsh, rv);
+  out.d = rebuild(d, dkeep, dsh, 1);
+  return out;
+}
+
+SvdResult svd_truncated(const BTensor& t, size_t split, double tol, size_t min_size, size_t max_size, double pw,
+                        SvdStats* stats) {
+  return truncate(svd(t, split, stats), max_size, min_size, tol, pw);
+}
+
+}  // namespace qtt

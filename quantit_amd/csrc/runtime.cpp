@@ -1,0 +1,138 @@
+#include "runtime.hpp"
+
+#include <cstdio>
+
+namespace qtt {
+
+std::string charge_str(const Groups& g, const Charge& c) {
+  std::string s = "(";
+  for (int i = 0; i < g.n; ++i) {
+    if (i) s += ",";
+    s += std::to_string(c.v[i]);
+  }
+  return s + ")";
+}
+
+static constexpr size_t kAlign = 256;
+static size_t round_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+Arena::Arena(size_t bytes) {
+  cap_ = round_up(bytes);
+  QTT_HIP(hipMalloc(&base_, cap_));
+  free_[0] = cap_;
+}
+
+Arena::~Arena() {
+  if (base_) (void)hipFree(base_);
+}
+
+size_t Arena::alloc(size_t bytes) {
+  size_t need = round_up(bytes ? bytes : 1);
+  std::lock_guard<std::mutex> lk(mu_);
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second >= need) {
+      size_t off = it->first;
+      size_t rest = it->second - need;
+      free_.erase(it);
+      if (rest) free_[off + need] = rest;
+      live_[off] = need;
+      used_ += need;
+      if (used_ > peak_) peak_ = used_;
+      return off;
+    }
+  }
+  throw OutOfMemory("device arena exhausted: requested " + std::to_string(need) + " bytes, " +
+                    std::to_string(cap_ - used_) + " free of " + std::to_string(cap_));
+}
+
+void Arena::release(size_t off) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto lv = live_.find(off);
+  if (lv == live_.end()) return;
+  size_t sz = lv->second;
+  live_.erase(lv);
+  used_ -= sz;
+  auto nx = free_.lower_bound(off);
+  // merge with the next free range
+  if (nx != free_.end() && nx->first == off + sz) {
+    sz += nx->second;
+    nx = free_.erase(nx);
+  }
+  // merge with the previous free range
+  if (nx != free_.begin()) {
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == off) {
+      pv->second += sz;
+      return;
+    }
+  }
+  free_[off] = sz;
+}
+
+Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(device) {
+  QTT_HIP(hipSetDevice(device));
+  QTT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  arena_ = std::make_unique<Arena>(arena_bytes);
+  stage_cap_ = round_up(stage_bytes);
+  QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_cap_, hipHostMallocDefault));
+  dev_stage_ = alloc(stage_cap_);
+}
+
+Context::~Context() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  pending_.clear();
+  for (auto e : event_pool_) (void)hipEventDestroy(e);
+  dev_stage_.reset();
+  arena_.reset();
+  if (host_stage_) (void)hipHostFree(host_stage_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void* Context::upload(const void* data, size_t bytes) {
+  size_t need = round_up(bytes ? bytes : 1);
+  if (need > stage_cap_) throw OutOfMemory("descriptor upload larger than the staging ring");
+  if (stage_pos_ + need > stage_cap_) {
+    // wrap: everything previously staged must have been consumed
+    QTT_HIP(hipStreamSynchronize(stream_));
+    stage_pos_ = 0;
+  }
+  char* h = host_stage_ + stage_pos_;
+  char* d = dev_stage_->cptr() + stage_pos_;
+  if (bytes) {
+    std::memcpy(h, data, bytes);
+    QTT_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
+  }
+  stage_pos_ += need;
+  return d;
+}
+
+void Context::sync() { QTT_HIP(hipStreamSynchronize(stream_)); }
+
+hipEvent_t Context::event() {
+  if (event_next_ == event_pool_.size()) {
+    hipEvent_t e;
+    QTT_HIP(hipEventCreate(&e));
+    event_pool_.push_back(e);
+  }
+  return event_pool_[event_next_++];
+}
+
+void Context::record(const char* name, hipEvent_t a, hipEvent_t b) {
+  pending_.push_back({name, a, b});
+  if (pending_.size() > 4096) flush_timings();
+}
+
+void Context::flush_timings() {
+  QTT_HIP(hipStreamSynchronize(stream_));
+  for (auto& p : pending_) {
+    float ms = 0.f;
+    QTT_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& e = kernel_ms[p.name];
+    e.first += ms;
+    e.second += 1;
+  }
+  pending_.clear();
+  event_next_ = 0;
+}
+
+}  // namespace qtt

@@ -1,0 +1,127 @@
+// Device runtime: the HBM arena that owns all block data of one context, the
+// context (device, HIP stream, arena, descriptor staging) and timing hooks.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+
+#include "qtt_common.hpp"
+
+namespace qtt {
+
+// One hipMalloc'd region suballocated first-fit with 256-byte alignment and
+// coalescing of free neighbours.  Allocation is per tensor (all blocks of a
+// tensor are contiguous), so the free list stays short.
+class Arena {
+ public:
+  explicit Arena(size_t bytes);
+  ~Arena();
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
+
+  size_t alloc(size_t bytes);  // byte offset from base(); throws OutOfMemory
+  void release(size_t off);
+  char* base() const { return base_; }
+  size_t capacity() const { return cap_; }
+  size_t used() const { return used_; }
+  size_t peak() const { return peak_; }
+  void reset_peak() { peak_ = used_; }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0, peak_ = 0;
+  std::map<size_t, size_t> free_;            // offset -> size
+  std::unordered_map<size_t, size_t> live_;  // offset -> size
+  std::mutex mu_;
+};
+
+struct DeviceBuffer {
+  Arena* arena;
+  size_t off;
+  size_t bytes;
+  DeviceBuffer(Arena* a, size_t nbytes) : arena(a), off(nbytes ? a->alloc(nbytes) : 0), bytes(nbytes) {}
+  ~DeviceBuffer() {
+    if (bytes) arena->release(off);
+  }
+  double* dptr() const { return reinterpret_cast<double*>(arena->base() + off); }
+  char* cptr() const { return arena->base() + off; }
+};
+using BufPtr = std::shared_ptr<DeviceBuffer>;
+
+enum class Precision : int {
+  FP64 = 0,        // FP64 MFMA contractions
+  SPLIT_FP32 = 1,  // hi/lo split FP32 MFMA, compensated, recombined in FP64
+};
+
+struct KernelTimer;
+
+class Context {
+ public:
+  Context(int device, size_t arena_bytes, size_t stage_bytes = size_t(64) << 20);
+  ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  Arena& arena() { return *arena_; }
+  BufPtr alloc(size_t bytes) { return std::make_shared<DeviceBuffer>(arena_.get(), bytes); }
+
+  // Copies host descriptors to device memory in stream order; returns a device
+  // pointer valid for kernels enqueued afterwards on this context's stream.
+  void* upload(const void* data, size_t bytes);
+  template <class T>
+  T* upload_vec(const std::vector<T>& v) {
+    return static_cast<T*>(upload(v.data(), v.size() * sizeof(T)));
+  }
+  void sync();
+
+  Precision precision = Precision::FP64;
+  // Reference-shaped baseline: one GEMM launch per block pair (mm, addmm_).
+  bool per_block_launch = false;
+  // Per-kernel accumulated time is recorded when profiling is on.
+  bool profiling = false;
+  std::map<std::string, std::pair<double, long>> kernel_ms;  // name -> (ms, launches)
+  void record(const char* name, hipEvent_t a, hipEvent_t b);
+  hipEvent_t event();
+  void flush_timings();  // synchronizes and folds pending event pairs into kernel_ms
+
+ private:
+  int device_;
+  hipStream_t stream_ = nullptr;
+  std::unique_ptr<Arena> arena_;
+  char* host_stage_ = nullptr;
+  BufPtr dev_stage_;
+  size_t stage_cap_ = 0, stage_pos_ = 0;
+  std::vector<hipEvent_t> event_pool_;
+  size_t event_next_ = 0;
+  struct Pending {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending_;
+};
+
+// RAII helper: brackets one launch with events when profiling is enabled.
+struct KernelTimer {
+  Context& ctx;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  KernelTimer(Context& c, const char* n) : ctx(c), name(n) {
+    if (ctx.profiling) {
+      a = ctx.event();
+      b = ctx.event();
+      QTT_HIP(hipEventRecord(a, ctx.stream()));
+    }
+  }
+  ~KernelTimer() {
+    if (ctx.profiling) {
+      (void)hipEventRecord(b, ctx.stream());
+      ctx.record(name, a, b);
+    }
+  }
+};
+
+}  // namespace qtt

@@ -1,0 +1,385 @@
+// Block-sparse contraction: host-side sector matching that reproduces the
+// output block list and accumulation order of btensor::tensordot
+// (sources/btensor.cpp:1971-2119), and one grouped-GEMM launch on device.
+#include <algorithm>
+#include <numeric>
+
+#include "btensor.hpp"
+#include "kernels.hpp"
+
+namespace qtt {
+
+// check_product_compat<Throws> (sources/btensor.cpp:782-825)
+bool check_product_compat(const Shape& a, const Shape& b, const std::vector<int64_t>& da,
+                          const std::vector<int64_t>& db, bool throws) {
+  if (da.size() != db.size()) {
+    if (throws) throw CheckError("both dimension lists should have the same length.");
+    return false;
+  }
+  for (size_t i = 0; i < da.size(); ++i) {
+    const int x = int(da[i]), y = int(db[i]);
+    if (a.nsec(x) != b.nsec(y)) {
+      if (throws)
+        throw CheckError("contracted dimensions need to match, but first has " + std::to_string(a.nsec(x)) +
+                         " sections along dim " + std::to_string(x) + "  and second has " +
+                         std::to_string(b.nsec(y)) + " sections along dim " + std::to_string(y));
+      return false;
+    }
+    for (int s = 0; s < a.nsec(x); ++s) {
+      if (q_add(a.g, a.cvals[x][s], b.cvals[y][s]) != q_neutral()) {
+        if (throws)
+          throw CheckError("contracted conserved numbers need to sum to zero, but there is a violation when "
+                           "contracting dim " +
+                           std::to_string(x) + " of the left tensor with dim " + std::to_string(y) +
+                           " of the right tensor");
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
+namespace {
+
+// Element strides of a block's matrix view over the listed dims (in order);
+// size-1 dims are skipped.  Returns false when the dims do not collapse into
+// a single stride (the block then needs a packing copy).
+bool collapse(const int64_t* dims, const int64_t* st, const std::vector<int>& which, int64_t& stride) {
+  int last = -1;
+  for (int d : which) {
+    if (dims[d] == 1) continue;
+    if (last >= 0 && st[last] != st[d] * dims[d]) return false;
+    last = d;
+  }
+  stride = last >= 0 ? st[last] : 1;
+  return true;
+}
+
+void add_chunks(std::vector<kern::ChunkTask>& tasks, int desc, int64_t numel) {
+  const int64_t nch = (numel + kern::kChunk - 1) / kern::kChunk;
+  for (int64_t c = 0; c < nch; ++c) tasks.push_back({desc, int32_t(c)});
+}
+
+struct KeyEnt {
+  BIdx key;
+  int src;
+};
+
+}  // namespace
+
+BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
+                  const std::vector<int64_t>& db_in, ContractStats* stats) {
+  Context* ctx = a.ctx;
+  if (a.shape.g != b.shape.g) throw CheckError("the two tensors have different type of conserved quantities");
+  if (da_in.size() != db_in.size()) throw CheckError("both dimension lists should have the same length.");
+  const int ra = a.rank(), rb = b.rank();
+  std::vector<int64_t> da(da_in), db(db_in);
+  for (auto& d : da) {
+    if (d < 0) d += ra;
+    if (d < 0 || d >= ra) throw CheckError("dimension out of range");
+  }
+  for (auto& d : db) {
+    if (d < 0) d += rb;
+    if (d < 0 || d >= rb) throw CheckError("dimension out of range");
+  }
+  check_product_compat(a.shape, b.shape, da, db, true);
+  const int dl = int(da.size());
+  std::vector<int> free_a, free_b, p1, p2prime;
+  for (int i = 0; i < ra; ++i)
+    if (std::find(da.begin(), da.end(), i) == da.end()) free_a.push_back(i);
+  for (int i = 0; i < rb; ++i)
+    if (std::find(db.begin(), db.end(), i) == db.end()) free_b.push_back(i);
+  p1 = free_a;
+  for (auto d : da) p1.push_back(int(d));
+  p2prime = free_b;
+  for (auto d : db) p2prime.push_back(int(d));
+  const int na = int(free_a.size()), nb = int(free_b.size());
+  if (na + nb > kMaxRank) throw InvalidArgument("contraction result rank too large");
+
+  Shape os;
+  os.g = a.shape.g;
+  for (int d : free_a) {
+    os.secs.push_back(a.shape.secs[d]);
+    os.cvals.push_back(a.shape.cvals[d]);
+  }
+  for (int d : free_b) {
+    os.secs.push_back(b.shape.secs[d]);
+    os.cvals.push_back(b.shape.cvals[d]);
+  }
+  os.sel = q_add(os.g, a.shape.sel, b.shape.sel);
+  BTensor out(ctx, os);
+
+  // permuted block keys (permute_bl, sources/btensor.cpp:1843-1894), sorted
+  std::vector<KeyEnt> t1(a.blocks.size()), t2(b.blocks.size());
+  for (size_t k = 0; k < a.blocks.size(); ++k) {
+    t1[k].src = int(k);
+    for (int j = 0; j < ra; ++j) t1[k].key.i[j] = a.blocks[k].idx.i[p1[j]];
+  }
+  for (size_t k = 0; k < b.blocks.size(); ++k) {
+    t2[k].src = int(k);
+    for (int j = 0; j < rb; ++j) t2[k].key.i[j] = b.blocks[k].idx.i[p2prime[j]];
+  }
+  std::sort(t1.begin(), t1.end(), [ra](const KeyEnt& x, const KeyEnt& y) { return bidx_less(x.key, y.key, ra); });
+  std::sort(t2.begin(), t2.end(), [rb](const KeyEnt& x, const KeyEnt& y) { return bidx_less(x.key, y.key, rb); });
+
+  auto prefix_eq = [](const BIdx& x, const BIdx& y, int n) {
+    for (int j = 0; j < n; ++j)
+      if (x.i[j] != y.i[j]) return false;
+    return true;
+  };
+  // group index of every t2 entry (runs of equal free prefix)
+  std::vector<int> g2of(t2.size());
+  {
+    int g = -1;
+    for (size_t k = 0; k < t2.size(); ++k) {
+      if (k == 0 || !prefix_eq(t2[k].key, t2[k - 1].key, nb)) ++g;
+      g2of[k] = g;
+    }
+  }
+  // t2 entries ordered by (contracted suffix, group): suffix lookup table
+  std::vector<int> by_suffix(t2.size());
+  std::iota(by_suffix.begin(), by_suffix.end(), 0);
+  auto suffix_less2 = [&](int x, int y) {
+    for (int j = 0; j < dl; ++j) {
+      const int u = t2[x].key.i[nb + j], v = t2[y].key.i[nb + j];
+      if (u != v) return u < v;
+    }
+    return g2of[x] < g2of[y];
+  };
+  std::sort(by_suffix.begin(), by_suffix.end(), suffix_less2);
+
+  struct Match {
+    int g2, e1, e2;
+  };
+  struct OutBlk {
+    BIdx idx;
+    std::vector<std::pair<int, int>> pairs;  // (a block, b block)
+  };
+  std::vector<OutBlk> outs;
+  std::vector<Match> matches;
+  size_t s1 = 0;
+  while (s1 < t1.size()) {
+    size_t e1 = s1 + 1;
+    while (e1 < t1.size() && prefix_eq(t1[e1].key, t1[s1].key, na)) ++e1;
+    matches.clear();
+    for (size_t k = s1; k < e1; ++k) {
+      // range of t2 entries whose suffix equals this entry's suffix
+      auto cmp_lo = [&](int x) {
+        for (int j = 0; j < dl; ++j) {
+          const int u = t2[x].key.i[nb + j], v = t1[k].key.i[na + j];
+          if (u != v) return u < v;
+        }
+        return false;
+      };
+      auto lo = std::partition_point(by_suffix.begin(), by_suffix.end(), cmp_lo);
+      for (auto it = lo; it != by_suffix.end(); ++it) {
+        bool eq = true;
+        for (int j = 0; j < dl && eq; ++j) eq = t2[*it].key.i[nb + j] == t1[k].key.i[na + j];
+        if (!eq) break;
+        matches.push_back({g2of[*it], int(k), *it});
+      }
+    }
+    std::stable_sort(matches.begin(), matches.end(), [](const Match& x, const Match& y) { return x.g2 < y.g2; });
+    size_t m = 0;
+    while (m < matches.size()) {
+      size_t me = m + 1;
+      while (me < matches.size() && matches[me].g2 == matches[m].g2) ++me;
+      OutBlk ob;
+      for (int j = 0; j < na; ++j) ob.idx.i[j] = t1[s1].key.i[j];
+      for (int j = 0; j < nb; ++j) ob.idx.i[na + j] = t2[matches[m].e2].key.i[j];
+      for (size_t q = m; q < me; ++q) ob.pairs.push_back({t1[matches[q].e1].src, t2[matches[q].e2].src});
+      outs.push_back(std::move(ob));
+      m = me;
+    }
+    s1 = e1;
+  }
+
+  std::vector<BIdx> oidx;
+  oidx.reserve(outs.size());
+  for (auto& o : outs) oidx.push_back(o.idx);
+  out.allocate(oidx);
+  if (outs.empty()) return out;
+
+  // matrix views of the operand blocks; blocks whose permuted view is not a
+  // plain strided matrix are packed (row-major) into a scratch buffer
+  double* ab = reinterpret_cast<double*>(ctx->arena().base());
+  struct View {
+    int64_t off = -1, rs = 0, cs = 0;
+    bool need_pack = false;
+    int64_t M = 0, K = 0;
+  };
+  std::vector<View> va(a.blocks.size()), vb(b.blocks.size());
+  std::vector<char> used_a(a.blocks.size(), 0), used_b(b.blocks.size(), 0);
+  for (auto& o : outs)
+    for (auto& p : o.pairs) {
+      used_a[p.first] = 1;
+      used_b[p.second] = 1;
+    }
+  std::vector<int> ca(da.begin(), da.end()), cb(db.begin(), db.end());
+  std::vector<kern::PermDesc> pdesc;
+  std::vector<kern::ChunkTask> ptasks;
+  int64_t pack_elems = 0;
+  std::vector<std::pair<int64_t*, int64_t>> fixups;  // (pointer to off, element offset inside scratch)
+  auto plan_view = [&](const BTensor& t, const Block& blk, const std::vector<int>& rows, const std::vector<int>& cols,
+                       View& v) {
+    const int r = t.rank();
+    int64_t dims[kMaxRank], st[kMaxRank];
+    t.shape.block_dims(blk.idx, dims);
+    if (r) st[r - 1] = 1;
+    for (int i = r - 2; i >= 0; --i) st[i] = st[i + 1] * dims[i + 1];
+    v.M = 1;
+    v.K = 1;
+    for (int d : rows) v.M *= dims[d];
+    for (int d : cols) v.K *= dims[d];
+    int64_t rs, cs;
+    if (collapse(dims, st, rows, rs) && collapse(dims, st, cols, cs)) {
+      v.off = t.ptr(blk) - ab;
+      v.rs = rs;
+      v.cs = cs;
+      // a unit-length side: keep the canonical contiguous stride
+      if (v.M == 1) v.rs = v.K;
+      if (v.K == 1) v.cs = 1;
+      if (v.M == 1 && v.K > 1) v.cs = cs;
+      return;
+    }
+    v.need_pack = true;
+    kern::PermDesc d{};
+    d.src_off = t.ptr(blk) - ab;
+    d.dst_off = pack_elems;  // fixed up once the scratch is allocated
+    d.numel = blk.numel;
+    d.rank = int32_t(rows.size() + cols.size());
+    int j = 0;
+    for (int dd : rows) {
+      d.dims[j] = int32_t(dims[dd]);
+      d.sstride[j++] = st[dd];
+    }
+    for (int dd : cols) {
+      d.dims[j] = int32_t(dims[dd]);
+      d.sstride[j++] = st[dd];
+    }
+    v.off = pack_elems;
+    v.rs = v.K;
+    v.cs = 1;
+    add_chunks(ptasks, int(pdesc.size()), blk.numel);
+    pdesc.push_back(d);
+    pack_elems += blk.numel;
+  };
+  for (size_t k = 0; k < a.blocks.size(); ++k)
+    if (used_a[k]) plan_view(a, a.blocks[k], free_a, ca, va[k]);
+  for (size_t k = 0; k < b.blocks.size(); ++k)
+    if (used_b[k]) plan_view(b, b.blocks[k], cb, free_b, vb[k]);
+  BufPtr scratch;
+  if (pack_elems) {
+    scratch = ctx->alloc(size_t(pack_elems) * sizeof(double));
+    const int64_t soff = scratch->dptr() - ab;
+    for (auto& d : pdesc) d.dst_off += soff;
+    for (auto& v : va)
+      if (v.need_pack) v.off += soff;
+    for (auto& v : vb)
+      if (v.need_pack) v.off += soff;
+    KernelTimer kt(*ctx, "pack_permute");
+    auto* dd = ctx->upload_vec(pdesc);
+    auto* dt = ctx->upload_vec(ptasks);
+    kern::permute_copy(ctx->stream(), ab, dd, dt, int(ptasks.size()));
+  }
+
+  // grouped GEMM descriptors
+  std::vector<kern::GemmOut> gouts(outs.size());
+  std::vector<kern::GemmPair> gpairs;
+  std::vector<kern::GemmTile> tiles;
+  std::vector<int64_t> tile_work;
+  double flops = 0, bytes = 0;
+  for (size_t o = 0; o < outs.size(); ++o) {
+    const Block& cb_ = out.blocks[o];
+    int64_t odims[kMaxRank];
+    out.shape.block_dims(cb_.idx, odims);
+    int64_t M = 1, N = 1;
+    for (int j = 0; j < na; ++j) M *= odims[j];
+    for (int j = 0; j < nb; ++j) N *= odims[na + j];
+    gouts[o].c_off = out.ptr(cb_) - ab;
+    gouts[o].M = int32_t(M);
+    gouts[o].N = int32_t(N);
+    gouts[o].pair_begin = int32_t(gpairs.size());
+    int64_t ksum = 0;
+    for (auto& p : outs[o].pairs) {
+      const View& x = va[p.first];
+      const View& y = vb[p.second];
+      kern::GemmPair gp{};
+      gp.a_off = x.off;
+      gp.a_rs = x.rs;
+      gp.a_cs = x.cs;
+      gp.b_off = y.off;
+      gp.b_rs = y.rs;
+      gp.b_cs = y.cs;
+      gp.K = int32_t(x.K);
+      gpairs.push_back(gp);
+      ksum += x.K;
+      flops += 2.0 * double(M) * double(N) * double(x.K);
+    }
+    gouts[o].pair_end = int32_t(gpairs.size());
+    const int tm = int((M + kern::kGemmTile - 1) / kern::kGemmTile);
+    const int tn = int((N + kern::kGemmTile - 1) / kern::kGemmTile);
+    for (int i = 0; i < tm; ++i)
+      for (int j = 0; j < tn; ++j) {
+        tiles.push_back({int32_t(o), int16_t(i), int16_t(j)});
+        tile_work.push_back(ksum);
+      }
+    bytes += 8.0 * double(cb_.numel);
+  }
+  for (size_t k = 0; k < a.blocks.size(); ++k)
+    if (used_a[k]) bytes += 8.0 * double(a.blocks[k].numel);
+  for (size_t k = 0; k < b.blocks.size(); ++k)
+    if (used_b[k]) bytes += 8.0 * double(b.blocks[k].numel);
+  // longest work first for a balanced tail
+  {
+    std::vector<int> ord(tiles.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return tile_work[x] > tile_work[y]; });
+    std::vector<kern::GemmTile> st(tiles.size());
+    for (size_t i = 0; i < ord.size(); ++i) st[i] = tiles[ord[i]];
+    tiles.swap(st);
+  }
+  if (stats) {
+    stats->flops += flops;
+    stats->bytes += bytes;
+    stats->out_blocks += long(outs.size());
+    stats->pairs += long(gpairs.size());
+    stats->tiles += long(tiles.size());
+  }
+  auto* d_outs = ctx->upload_vec(gouts);
+  auto* d_pairs = ctx->upload_vec(gpairs);
+  if (ctx->per_block_launch) {
+    // reference-shaped baseline: one launch per block pair (mm, then addmm_)
+    std::vector<kern::GemmOut> one;
+    for (size_t o = 0; o < outs.size(); ++o) {
+      const int tm = (gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
+      const int tn = (gouts[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
+      std::vector<kern::GemmTile> tl;
+      for (int i = 0; i < tm; ++i)
+        for (int j = 0; j < tn; ++j) tl.push_back({0, int16_t(i), int16_t(j)});
+      if (tl.empty()) continue;
+      auto* d_tl = ctx->upload_vec(tl);
+      for (int p = gouts[o].pair_begin; p < gouts[o].pair_end; ++p) {
+        kern::GemmOut g1 = gouts[o];
+        g1.pair_begin = p;
+        g1.pair_end = p + 1;
+        auto* d_g1 = ctx->upload(&g1, sizeof(g1));
+        KernelTimer kt(*ctx, "gemm_per_block");
+        kern::gemm_f64(ctx->stream(), ab, d_tl, int(tl.size()), static_cast<kern::GemmOut*>(d_g1), d_pairs, 1.0,
+                       p == gouts[o].pair_begin ? 0.0 : 1.0);
+      }
+    }
+    return out;
+  }
+  auto* d_tiles = ctx->upload_vec(tiles);
+  if (ctx->precision == Precision::SPLIT_FP32) {
+    KernelTimer kt(*ctx, "gemm_split");
+    kern::gemm_split(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
+  } else {
+    KernelTimer kt(*ctx, "gemm_f64");
+    kern::gemm_f64(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
+  }
+  return out;
+}
+
+}  // namespace qtt

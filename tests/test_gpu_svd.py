@@ -1,0 +1,141 @@
+"""Parity of the device block SVD / eigh / truncation with the oracle's
+restatement of btensor_linalg.cpp.  Structure (sectors, charges, kept sizes,
+block sets) bit-exact; singular values within 1e-12 of the largest singular
+value; U and V compared through U.diag(d).V^T and projectors (gauge-free);
+orthonormality within 1e-12."""
+import numpy as np
+import pytest
+
+from oracle.btensor_oracle import (BTensor, eigh_split, svd_split, svd_truncated, tensordot, mul, rand_like)
+from tests.helpers import assert_same_structure, from_dev, inv_leg, rand_btensor, rand_leg, to_dev
+
+pytestmark = pytest.mark.gpu
+
+G1 = (0,)
+
+
+def _rank2(rng, nrow, ncol, smax, sel=(0,)):
+    rows = rand_leg(rng, nrow, lo=-4, hi=4, smax=smax)
+    cols = rand_leg(rng, ncol, lo=-4, hi=4, smax=smax)
+    return rand_btensor(rng, G1, [rows, cols], sel)
+
+
+def _check_svd(A, U, d, V, split):
+    """U.diag(d).V^T == A and column orthonormality, in dense form."""
+    Ad = A.to_dense()
+    m = int(np.prod(Ad.shape[:split]))
+    Ud = U.to_dense().reshape(m, -1)
+    Vd = V.to_dense().reshape(-1, Ud.shape[1])
+    dd = np.concatenate([d.blocks[(i,)] if (i,) in d.blocks else np.zeros(d.secs[0][i])
+                         for i in range(len(d.secs[0]))])
+    rec = (Ud * dd) @ Vd.T
+    scale = max(np.linalg.norm(Ad), 1e-300)
+    assert np.linalg.norm(rec - Ad.reshape(m, -1)) / scale < 1e-12
+    keep = dd > dd.max() * 1e-10 if dd.size else dd
+    Uk = Ud[:, keep]
+    Vk = Vd[:, keep]
+    assert np.abs(Uk.T @ Uk - np.eye(Uk.shape[1])).max() < 1e-12
+    assert np.abs(Vk.T @ Vk - np.eye(Vk.shape[1])).max() < 1e-12
+    return dd
+
+
+@pytest.mark.parametrize("seed,nrow,ncol,smax", [(0, 4, 4, 6), (1, 6, 3, 12), (2, 3, 7, 20), (3, 5, 5, 90)])
+def test_svd_rank2_matches_oracle(ctx, seed, nrow, ncol, smax):
+    import quantit_amd as qa
+    rng = np.random.default_rng(seed)
+    A = _rank2(rng, nrow, ncol, smax)
+    rU, rd, rV = svd_split(A, 1)
+    U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, A), 1))
+    assert_same_structure(d, rd)
+    assert_same_structure(U, rU)
+    assert_same_structure(V, rV)
+    smax_all = max(float(b.max()) for b in rd.blocks.values())
+    for k in rd.blocks:
+        assert np.abs(d.blocks[k] - rd.blocks[k]).max() <= 1e-12 * smax_all
+    dd = _check_svd(A, U, d, V, 1)
+    # singular values equal those of the dense matrix (library routine)
+    s_dense = np.linalg.svd(A.to_dense(), compute_uv=False)
+    top = np.sort(dd)[::-1][:len(s_dense)]
+    assert np.abs(top - s_dense[:len(top)]).max() <= 1e-12 * s_dense[0]
+
+
+def test_svd_rank4_split2(ctx):
+    import quantit_amd as qa
+    rng = np.random.default_rng(5)
+    l = rand_leg(rng, 4, smax=8)
+    s = [(1, (1,)), (1, (-1,))]
+    r = rand_leg(rng, 4, smax=8)
+    theta = rand_btensor(rng, G1, [l, s, s, r], (0,))
+    rU, rd, rV = svd_split(theta, 2)
+    U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, theta), 2))
+    assert_same_structure(U, rU)
+    assert_same_structure(V, rV)
+    assert_same_structure(d, rd)
+    _check_svd(theta, U, d, V, 2)
+    # projector onto the kept left space equals the oracle's
+    Pu = U.to_dense().reshape(-1, sum(U.secs[2]))
+    Pr = rU.to_dense().reshape(-1, sum(rU.secs[2]))
+    assert np.abs(Pu @ Pu.T - Pr @ Pr.T).max() < 1e-11
+
+
+def _spectral(rng, legs, sel, decay):
+    """Rank-2 tensor whose sector spectra are well separated geometric series
+    (no ties near any cut), built as U diag(s) V^T per sector."""
+    A = rand_btensor(rng, G1, legs, sel)
+    U, d, V = svd_split(A, 1)
+    sc = 1.0
+    for k in sorted(d.blocks):
+        n = d.blocks[k].shape[0]
+        d.blocks[k] = sc * decay ** np.arange(n) * (1 + 0.01 * rng.random())
+        sc *= 0.37
+    return tensordot(mul(U, d), V.conj(), [1], [1])
+
+
+@pytest.mark.parametrize("tol,mn,mx", [(1e-3, 1, 10 ** 9), (1e-6, 1, 7), (1e-2, 5, 10 ** 9), (1e-8, 2, 3)])
+def test_truncated_svd_structure_exact(ctx, tol, mn, mx):
+    import quantit_amd as qa
+    rng = np.random.default_rng(21)
+    rows = rand_leg(rng, 5, lo=-4, hi=4, smax=6)
+    A = _spectral(rng, [rows, inv_leg(G1, rows)], (0,), 0.3)
+    rU, rd, rV = svd_truncated(A, 1, tol, mn, mx)
+    U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, A), 1, tol=tol, min_size=mn, max_size=mx))
+    assert_same_structure(d, rd)
+    assert_same_structure(U, rU)
+    assert_same_structure(V, rV)
+    for k in rd.blocks:
+        assert np.abs(d.blocks[k] - rd.blocks[k]).max() <= 1e-12
+
+
+def test_svd_rank_deficient_orthonormal(ctx):
+    """Sectors with exactly zero singular values: U and V stay orthonormal."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(31)
+    rows = rand_leg(rng, 3, smax=8)
+    A = rand_btensor(rng, G1, [rows, inv_leg(G1, rows)], (0,))
+    for k in A.blocks:
+        b = A.blocks[k]
+        if b.shape[0] > 1:
+            b[-1] = b[0]  # duplicated row: rank deficient
+    U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, A), 1))
+    for t in (U, V):
+        M = t.to_dense()
+        assert np.abs(M.T @ M - np.eye(M.shape[1])).max() < 1e-12
+
+
+def test_eigh_matches_dense(ctx):
+    import quantit_amd as qa
+    rng = np.random.default_rng(41)
+    rows = [(5, (0,)), (3, (2,)), (7, (-2,)), (40, (4,))]
+    A = rand_btensor(rng, G1, [rows, inv_leg(G1, rows)], (0,))
+    for k in A.blocks:
+        A.blocks[k] = A.blocks[k] + A.blocks[k].T
+    rd, rU = eigh_split(A, 1)
+    d, U = (from_dev(x) for x in qa.eigh(to_dev(ctx, A), 1))
+    assert_same_structure(d, rd)
+    assert_same_structure(U, rU)
+    nrm = np.linalg.norm(A.to_dense(), 2)
+    for k in rd.blocks:
+        assert np.abs(d.blocks[k] - rd.blocks[k]).max() <= 1e-12 * nrm
+    Ud = U.to_dense()
+    dd = np.concatenate([d.blocks[(i,)] for i in range(len(d.secs[0]))])
+    assert np.abs(A.to_dense() @ Ud - Ud * dd).max() <= 1e-11 * nrm
