@@ -127,9 +127,10 @@ def longrange_heisenberg_dense(N, bonds, J=1.0, delta=1.0):
     triple per open bond origin that is still waiting for its partner."""
     sx, isy, sz, lo, idm = pauli()
     hi = lo.T
-    ops = {"p": 0.5 * hi, "m": 0.5 * lo, "z": 0.5 * sz}   # S+/S-/Sz with S=sigma/2
+    # Sx Sx + Sy Sy = (S+ S- + S- S+)/2 with S+- = hi/lo (unit entries); Sz = sz/2
+    ops = {"p": hi, "m": lo, "z": 0.5 * sz}
     partner = {"p": "m", "m": "p", "z": "z"}
-    coef = {"p": J, "m": J, "z": J * delta}
+    coef = {"p": 0.5 * J, "m": 0.5 * J, "z": J * delta}
     Ws = []
     # open channels after site i: origins a <= i with a partner j > i
     open_after = []

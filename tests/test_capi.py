@@ -1,0 +1,52 @@
+"""The C ABI library loads without a GPU and exports every function that
+include/quantit_amd.h declares; the Python mirror declares all of them.
+No compute call is made here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "quantit_amd.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qtt_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import quantit_amd as qa
+    if not qa.LIB_PATH.exists():
+        from quantit_amd.build import build
+        build()
+    return ctypes.CDLL(str(qa.LIB_PATH))
+
+
+def test_header_declares_the_boundary():
+    fns = header_functions()
+    for name in ["qtt_tensordot", "qtt_svd", "qtt_svd_truncated", "qtt_eigh", "qtt_truncate", "qtt_dmrg",
+                 "qtt_btensor_create", "qtt_context_create"]:
+        assert name in fns
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [f for f in header_functions() if not hasattr(lib, f)]
+    assert not missing, missing
+
+
+def test_python_mirror_declares_every_symbol():
+    import quantit_amd as qa
+    assert sorted(qa.SIGNATURES) == header_functions()
+
+
+def test_error_channel_without_gpu(lib):
+    """Argument errors come back as a status and a message, no exception crosses the ABI."""
+    lib.qtt_last_error.restype = ctypes.c_char_p
+    lib.qtt_btensor_info.restype = ctypes.c_int
+    st = lib.qtt_btensor_info(None, None, None, None, None, None)
+    assert st == 1
+    assert b"null pointer" in lib.qtt_last_error()
