@@ -15,11 +15,12 @@ namespace kern {
 // pairs of one output block taken in list order (the reference's mm +
 // addmm_ order, sources/btensor.cpp:2095-2102).  A_p[m,k] =
 // base[a_off + m*a_rs + k*a_cs], B_p[k,n] = base[b_off + k*b_rs + n*b_cs];
-// C is row-major M x N at c_off.
+// C is row-major M x N at c_off with row pitch ldc.
 struct GemmOut {
   int64_t c_off;
   int32_t M, N;
   int32_t pair_begin, pair_end;
+  int32_t ldc, pad;  // row pitch of C (N for a packed block)
 };
 struct GemmPair {
   int64_t a_off, b_off;
@@ -117,11 +118,32 @@ struct SvdSector {
 constexpr int kSvdBlock = 16;  // columns per Jacobi block
 void svd_init_w(hipStream_t s, double* base, const SvdSector* secs, int nsec);
 void svd_round(hipStream_t s, double* base, const SvdSector* secs, const int32_t* work, int nwork, int round,
-               int* counters, double tol);
-void svd_finalize(hipStream_t s, double* base, const SvdSector* secs, int nsec);
+               int* counters, double tol, int inner_sweeps);
+// norms + ranks (one work-group per sector), tiled transposed scatter of the
+// (sector, part, row0, col0) tasks into U / V, orthonormal completion.
+void svd_finalize(hipStream_t s, double* base, const SvdSector* secs, const SvdSector* host_secs, int nsec,
+                  const int4* tasks, int ntasks);
 // eigh mode: c = ||A||_F per sector, stored at shift_off, and Y += c I.
 void svd_shift(hipStream_t s, double* base, const SvdSector* secs, int nsec);
 void fill_i32(hipStream_t s, int32_t* p, int n, int32_t v);
+
+// Single-work-group one-sided Jacobi with the whole sector in LDS
+// (descriptors: see SmallJacobi in svd_jacobi.hip).
+struct SmallJacobi {
+  int64_t y_off, w_off;  // Y (ell x k, column-major, ld ldy), W (ldw x ldw, column-major)
+  int32_t ell, k, ldy, ldw;
+  int32_t flag_idx, pad;  // flags[flag_idx] = rotations of the first sweep (-1: none)
+};
+void jacobi_small(hipStream_t s, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
+                  size_t lds_bytes);
+size_t jacobi_small_lds_limit();
+// Eigenvectors of k x k Gram matrices (SmallJacobi with ell = k): W written
+// k x k at w_off with leading dimension ldw; flags[flag_idx] = rotations of
+// the first sweep (0: W = I).
+constexpr int kGramEigMaxK = 144;
+void gram_eig(hipStream_t s, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
+              size_t lds_bytes);
+size_t gram_eig_lds_bytes(int k);  // largest dynamic LDS per work-group (bytes)
 
 // Global truncation (sources/btensor_linalg.cpp:657-755,
 // sources/LinearAlgebra.cpp:57-75): singular values of nsec sectors stored

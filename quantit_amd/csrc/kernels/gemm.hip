@@ -93,6 +93,72 @@ __device__ __forceinline__ void load_tiles(const double* __restrict__ A, const d
   }
 }
 
+// Register-staged variant of load_tiles for the FP64 kernel: fetch() issues
+// the global loads of one (A, B) k-tile into registers, stash() writes them to
+// LDS, so the loads of tile t+1 overlap the MFMAs of tile t.
+struct Regs {
+  double a[4], b[4];
+};
+
+__device__ __forceinline__ void fetch(const double* __restrict__ base, const GemmPair& pr, int M, int N, int m0,
+                                      int n0, int k0, int tid, Regs& r) {
+  const double* A = base + pr.a_off;
+  const double* B = base + pr.b_off;
+  const int K = pr.K;
+  if (pr.a_cs == 1) {
+    const int m = tid >> 2, kq = (tid & 3) * 4, gm = m0 + m;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gk = k0 + kq + j;
+      r.a[j] = (gm < M && gk < K) ? A[int64_t(gm) * pr.a_rs + gk] : 0.0;
+    }
+  } else {
+    const int k = tid >> 4, mq = (tid & 15) * 4, gk = k0 + k;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gm = m0 + mq + j;
+      r.a[j] = (gm < M && gk < K) ? A[int64_t(gm) * pr.a_rs + int64_t(gk) * pr.a_cs] : 0.0;
+    }
+  }
+  if (pr.b_cs == 1) {
+    const int k = tid >> 4, nq = (tid & 15) * 4, gk = k0 + k;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gn = n0 + nq + j;
+      r.b[j] = (gn < N && gk < K) ? B[int64_t(gk) * pr.b_rs + gn] : 0.0;
+    }
+  } else {
+    const int n = tid >> 2, kq = (tid & 3) * 4, gn = n0 + n;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gk = k0 + kq + j;
+      r.b[j] = (gn < N && gk < K) ? B[int64_t(gk) * pr.b_rs + int64_t(gn) * pr.b_cs] : 0.0;
+    }
+  }
+}
+
+__device__ __forceinline__ void stash(const GemmPair& pr, int tid, const Regs& r, double (*As)[LDP],
+                                      double (*Bs)[LDP]) {
+  if (pr.a_cs == 1) {
+    const int m = tid >> 2, kq = (tid & 3) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) As[kq + j][m] = r.a[j];
+  } else {
+    const int k = tid >> 4, mq = (tid & 15) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) As[k][mq + j] = r.a[j];
+  }
+  if (pr.b_cs == 1) {
+    const int k = tid >> 4, nq = (tid & 15) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Bs[k][nq + j] = r.b[j];
+  } else {
+    const int n = tid >> 2, kq = (tid & 3) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Bs[kq + j][n] = r.b[j];
+  }
+}
+
 // Output element (row, col) of 16x16 MFMA tile register r in lane l:
 //   v_mfma_f64_16x16x4_f64: row = 4*r + l/16, col = l%16
 //   v_mfma_f32_16x16x4_f32: row = 4*(l/16) + r, col = l%16
@@ -109,7 +175,7 @@ __device__ __forceinline__ void store_tile(double* __restrict__ C, const GemmOut
         const int row = m0 + wm + i * 16 + (F64Layout ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r);
         const int col = n0 + wn + j * 16 + (lane & 15);
         if (row < o.M && col < o.N) {
-          double* p = C + int64_t(row) * o.N + col;
+          double* p = C + int64_t(row) * o.ldc + col;
           double v = alpha * acc[i][j][r];
           if (beta != 0.0) v += beta * (*p);
           *p = v;
@@ -133,13 +199,28 @@ __global__ __launch_bounds__(256) void k_gemm_f64(double* __restrict__ base, con
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  for (int p = o.pair_begin; p < o.pair_end; ++p) {
-    const GemmPair pr = pairs[p];
-    const double* A = base + pr.a_off;
-    const double* B = base + pr.b_off;
-    for (int k0 = 0; k0 < pr.K; k0 += BK) {
-      load_tiles<double>(A, B, pr, o.M, o.N, m0, n0, k0, tid, As, Bs, nullptr, nullptr);
+  // flattened (pair, k-tile) iteration with one tile of register prefetch
+  int p = o.pair_begin, k0 = 0;
+  while (p < o.pair_end && pairs[p].K <= 0) ++p;
+  if (p < o.pair_end) {
+    GemmPair pr = pairs[p];
+    Regs cur;
+    fetch(base, pr, o.M, o.N, m0, n0, 0, tid, cur);
+    while (true) {
+      stash(pr, tid, cur, As, Bs);
       __syncthreads();
+      // advance to the next tile and prefetch it
+      int np = p, nk = k0 + BK;
+      GemmPair npr = pr;
+      if (nk >= pr.K) {
+        nk = 0;
+        ++np;
+        while (np < o.pair_end && pairs[np].K <= 0) ++np;
+        if (np < o.pair_end) npr = pairs[np];
+      }
+      const bool more = np < o.pair_end;
+      Regs nxt;
+      if (more) fetch(base, npr, o.M, o.N, m0, n0, nk, tid, nxt);
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4), c = lane & 15;
@@ -151,6 +232,11 @@ __global__ __launch_bounds__(256) void k_gemm_f64(double* __restrict__ base, con
         acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
       }
       __syncthreads();
+      if (!more) break;
+      p = np;
+      k0 = nk;
+      pr = npr;
+      cur = nxt;
     }
   }
   store_tile<true>(base + o.c_off, o, m0, n0, wm, wn, lane, acc, alpha, beta);

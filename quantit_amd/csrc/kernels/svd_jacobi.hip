@@ -55,23 +55,28 @@ __device__ __forceinline__ int circle_block(int pos, int round, int nb) {
 __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, const int* cols,
                                double (*Ys)[YP], const double (*Q)[GP]) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = tid >> 3, lq = (tid & 7) * 8;
+  const double* src = M + int64_t(cols[c]) * ld;
+  double buf[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) buf[i] = (lq + i < rows) ? src[lq + i] : 0.0;
   for (int l0 = 0; l0 < rows; l0 += 64) {
-    {
-      const int c = tid >> 3, lq = (tid & 7) * 8;
-      const double* src = M + int64_t(cols[c]) * ld;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Ys[c][lq + i] = buf[i];
+    __syncthreads();
+    if (l0 + 64 < rows) {
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const int l = l0 + lq + i;
-        Ys[c][lq + i] = (l < rows) ? src[l] : 0.0;
+        const int l = l0 + 64 + lq + i;
+        buf[i] = (l < rows) ? src[l] : 0.0;
       }
     }
-    __syncthreads();
     v4d acc[2] = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
     for (int kk = 0; kk < NB2; kk += 4) {
-      const int kr = kk + (lane >> 4), c = lane & 15;
-      const double a = Ys[kr][wave * 16 + c];
-      const double b0 = Q[kr][c], b1 = Q[kr][16 + c];
+      const int kr = kk + (lane >> 4), cl = lane & 15;
+      const double a = Ys[kr][wave * 16 + cl];
+      const double b0 = Q[kr][cl], b1 = Q[kr][16 + cl];
       acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[0], 0, 0, 0);
       acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[1], 0, 0, 0);
     }
@@ -89,7 +94,7 @@ __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, con
 
 __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, const SvdSector* __restrict__ secs,
                                                    const int32_t* __restrict__ work, int round,
-                                                   int* __restrict__ counters, double tol) {
+                                                   int* __restrict__ counters, double tol, int inner_sweeps) {
   __shared__ double Ys[NB2][YP];
   __shared__ double G[2][NB2][GP];
   __shared__ double Q[2][NB2][GP];
@@ -115,23 +120,32 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   // ---- 1. Gram matrix ------------------------------------------------------
   const int wi = wave >> 1, wj = wave & 1;
   v4d g = v4d{0.0, 0.0, 0.0, 0.0};
-  for (int l0 = 0; l0 < ell; l0 += 64) {
-    {
-      const int c = tid >> 3, lq = (tid & 7) * 8;
-      const double* src = Y + int64_t(cols[c]) * ld;
+  {
+    // 64-row chunks staged through LDS; the next chunk's global loads are
+    // issued before the MFMAs of the current one
+    const int c = tid >> 3, lq = (tid & 7) * 8;
+    const double* src = Y + int64_t(cols[c]) * ld;
+    double buf[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int l = l0 + lq + i;
-        Ys[c][lq + i] = (l < ell) ? src[l] : 0.0;
+    for (int i = 0; i < 8; ++i) buf[i] = (lq + i < ell) ? src[lq + i] : 0.0;
+    for (int l0 = 0; l0 < ell; l0 += 64) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Ys[c][lq + i] = buf[i];
+      __syncthreads();
+      if (l0 + 64 < ell) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int l = l0 + 64 + lq + i;
+          buf[i] = (l < ell) ? src[l] : 0.0;
+        }
       }
-    }
-    __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 64; kk += 4) {
-      const int kr = kk + (lane >> 4), c = lane & 15;
-      g = __builtin_amdgcn_mfma_f64_16x16x4f64(Ys[wi * 16 + c][kr], Ys[wj * 16 + c][kr], g, 0, 0, 0);
+      for (int kk = 0; kk < 64; kk += 4) {
+        const int kr = kk + (lane >> 4), cl = lane & 15;
+        g = __builtin_amdgcn_mfma_f64_16x16x4f64(Ys[wi * 16 + cl][kr], Ys[wj * 16 + cl][kr], g, 0, 0, 0);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) G[0][wi * 16 + 4 * r + (lane >> 4)][wj * 16 + (lane & 15)] = g[r];
@@ -154,7 +168,7 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   }
   int cur = 0;
   __syncthreads();
-  for (int sweep = 0; sweep < 12; ++sweep) {
+  for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
     if (tid == 0) nrot = 0;
     __syncthreads();
     for (int r = 0; r < NB2 - 1; ++r) {
@@ -201,9 +215,10 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
 }
 
 void svd_round(hipStream_t st, double* base, const SvdSector* secs, const int32_t* work, int nwork, int round,
-               int* counters, double tol) {
+               int* counters, double tol, int inner_sweeps) {
   if (nwork <= 0) return;
-  hipLaunchKernelGGL(k_svd_round, dim3(nwork), dim3(256), 0, st, base, secs, work, round, counters, tol);
+  hipLaunchKernelGGL(k_svd_round, dim3(nwork), dim3(256), 0, st, base, secs, work, round, counters, tol,
+                     inner_sweeps);
 }
 
 // Block-wide reductions (fixed order, deterministic).
@@ -220,25 +235,344 @@ __device__ double wg_sum(double v, double* sh) {
   return r;
 }
 
-// One work-group per sector: singular values (column norms of Y), descending
-// order, normalised Z columns and W columns scattered to U and V (row-major,
-// m x k and n x k), and an orthonormal completion of the columns whose
-// singular value is numerically zero.  aux = base + w_off + kp*kp holds
-// 2*kp + ell doubles of scratch.
-__global__ __launch_bounds__(256) void k_svd_finalize(double* __restrict__ base, const SvdSector* __restrict__ secs) {
-  __shared__ double sh[256];
+// ---- single-work-group Jacobi for sectors that fit in LDS -------------------
+// One work-group per sector holds Y (ell x ke, ke = k rounded up to even) in
+// LDS and runs cyclic one-sided Jacobi sweeps (circle ordering over single
+// columns) until a sweep rotates nothing.  Each column pair is handled by a
+// group of gsz lanes that reduce the three dot products with shuffles and
+// rotate the pair in LDS; W (ldw x ldw, column-major, global) accumulates the
+// rotations.  Y is written back at the end; flags[flag_idx] receives the
+// number of rotations of the first sweep.
+constexpr int kSmallThreads = 1024;
+
+__global__ __launch_bounds__(kSmallThreads) void k_jacobi_small(double* __restrict__ base,
+                                                                const SmallJacobi* __restrict__ js,
+                                                                int* __restrict__ flags, double tol, int max_sweeps) {
+  extern __shared__ double lds[];
+  __shared__ int nrot[2];
+  __shared__ int first;
+  const SmallJacobi J = js[blockIdx.x];
+  const int ell = J.ell, k = J.k;
+  const int ke = k + (k & 1);
+  double* Yg = base + J.y_off;
+  double* W = base + J.w_off;
+  const int tid = threadIdx.x;
+  // load Y (padding column zero) and set W to the identity
+  for (int64_t e = tid; e < int64_t(ell) * ke; e += blockDim.x) {
+    const int r = int(e % ell), c = int(e / ell);
+    lds[e] = c < k ? Yg[int64_t(c) * J.ldy + r] : 0.0;
+  }
+  for (int64_t e = tid; e < int64_t(J.ldw) * J.ldw; e += blockDim.x)
+    W[e] = (e % J.ldw == e / J.ldw) ? 1.0 : 0.0;
+  if (tid == 0) {
+    first = -1;
+    nrot[0] = nrot[1] = 0;
+  }
+  __syncthreads();
+  const int npairs = ke / 2;
+  int gsz = 64;
+  while (gsz > 1 && gsz * npairs > int(blockDim.x)) gsz >>= 1;
+  const int pi = tid / gsz, li = tid % gsz, pstride = int(blockDim.x) / gsz;
+  for (int sweep = 0; sweep < max_sweeps && ke >= 2; ++sweep) {
+    int* cnt = &nrot[sweep & 1];
+    for (int r = 0; r < ke - 1; ++r) {
+      for (int p = pi; p < npairs; p += pstride) {
+        const int a0 = circle_block(p, r, ke), b0 = circle_block(ke - 1 - p, r, ke);
+        const int a = min(a0, b0), b = max(a0, b0);
+        double* ya = lds + int64_t(a) * ell;
+        double* yb = lds + int64_t(b) * ell;
+        double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll 4
+        for (int l = li; l < ell; l += gsz) {
+          const double x = ya[l], y = yb[l];
+          al += x * x;
+          be += y * y;
+          ga += x * y;
+        }
+        for (int off = gsz >> 1; off > 0; off >>= 1) {
+          al += __shfl_xor(al, off, gsz);
+          be += __shfl_xor(be, off, gsz);
+          ga += __shfl_xor(ga, off, gsz);
+        }
+        if (al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+#pragma unroll 4
+          for (int l = li; l < ell; l += gsz) {
+            const double x = ya[l], y = yb[l];
+            ya[l] = c * x - sn * y;
+            yb[l] = sn * x + c * y;
+          }
+          double* wa = W + int64_t(a) * J.ldw;
+          double* wb = W + int64_t(b) * J.ldw;
+#pragma unroll 4
+          for (int l = li; l < k; l += gsz) {
+            const double x = wa[l], y = wb[l];
+            wa[l] = c * x - sn * y;
+            wb[l] = sn * x + c * y;
+          }
+          if (li == 0) atomicAdd(cnt, 1);
+        }
+      }
+      __syncthreads();
+    }
+    // cnt is final after the last barrier; the other counter is cleared for
+    // the next sweep before anyone can touch it (it is only used after the
+    // next sweep's first barrier)
+    const int n = *cnt;
+    if (tid == 0) {
+      if (first < 0) first = n;
+      nrot[(sweep + 1) & 1] = 0;
+    }
+    __syncthreads();
+    if (n == 0) break;
+  }
+  for (int64_t e = tid; e < int64_t(ell) * k; e += blockDim.x) {
+    const int r = int(e % ell), c = int(e / ell);
+    Yg[int64_t(c) * J.ldy + r] = lds[e];
+  }
+  if (tid == 0 && J.flag_idx >= 0) flags[J.flag_idx] = first < 0 ? 0 : first;
+}
+
+void jacobi_small(hipStream_t st, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
+                  size_t lds_bytes) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_jacobi_small, dim3(n), dim3(kSmallThreads), lds_bytes, st, base,
+                     static_cast<const SmallJacobi*>(descs), flags, tol, max_sweeps);
+}
+
+// ---- eigenvectors of a Gram matrix by one-sided Jacobi without accumulation --
+// One work-group per k x k Gram matrix G (k <= kGramEigMaxK) held in LDS.  The
+// one-sided Jacobi drives X = G W to mutually orthogonal columns; since G is
+// symmetric positive semi-definite, X = W diag(lambda) at convergence, so W is
+// recovered as the normalised columns of X and no rotation is accumulated.
+// Rotations are made down to tol/16, so the columns of X end orthonormal well
+// inside tol whatever the accuracy of the small eigenpairs; exactly zero
+// columns are completed from the unit vector with the largest residual
+// against the others (two projection passes).
+//
+// Each column pair of a round is owned by gsz lanes that keep its rows in
+// registers (kEigR per lane): one LDS read and one write per element per round.
+//
+// flags[flag_idx] = number of column pairs of G that fail the tol test in the
+// first sweep; when it is 0 the input was already diagonal to tol, the sweeps
+// stop and W = I exactly.
+constexpr int kEigR = 18;
+
+__global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__ base,
+                                                            const SmallJacobi* __restrict__ js,
+                                                            int* __restrict__ flags, double tol, int max_sweeps) {
+  extern __shared__ double lds[];
+  __shared__ int nrot[2];
+  __shared__ int nviol, best;
+  const SmallJacobi J = js[blockIdx.x];
+  const int k = J.k;
+  const int ke = k + (k & 1);
+  const int tid = threadIdx.x;
+  double* X = lds;                         // k x ke, column-major
+  double* nrm = lds + int64_t(k) * ke;     // ke
+  double* vec = nrm + ke;                  // ke
+  double* hh = vec + ke;                   // ke
+  double* Gg = base + J.y_off;
+  double* Wg = base + J.w_off;
+  const int npairs = ke / 2;
+  int gsz = 64;
+  while (gsz > 1 && gsz * npairs > int(blockDim.x)) gsz >>= 1;
+  if (k <= 0) return;
+  if (k > kGramEigMaxK || (k + gsz - 1) / gsz > kEigR) {  // never launched so by the host
+    for (int64_t e = tid; e < int64_t(k) * k; e += blockDim.x)
+      Wg[(e / k) * J.ldw + e % k] = (e / k == e % k) ? 1.0 : 0.0;
+    if (tid == 0 && J.flag_idx >= 0) flags[J.flag_idx] = -1;
+    return;
+  }
+  for (int e = tid; e < k * ke; e += blockDim.x) {
+    const int r = e % k, c = e / k;
+    X[e] = c < k ? Gg[int64_t(c) * J.ldy + r] : 0.0;
+  }
+  if (tid == 0) {
+    nviol = 0;
+    nrot[0] = nrot[1] = 0;
+  }
+  __syncthreads();
+  const double tol_rot = tol * (1.0 / 16.0);
+  const int pi = tid / gsz, li = tid % gsz;
+  bool ident = k < 2;
+  for (int sweep = 0; sweep < max_sweeps && k >= 2; ++sweep) {
+    int* cnt = &nrot[sweep & 1];
+    for (int r = 0; r < ke - 1; ++r) {
+      if (pi < npairs) {
+        const int a0 = circle_block(pi, r, ke), b0 = circle_block(ke - 1 - pi, r, ke);
+        double* ya = X + min(a0, b0) * k;
+        double* yb = X + max(a0, b0) * k;
+        double xa[kEigR], xb[kEigR];
+#pragma unroll
+        for (int j = 0; j < kEigR; ++j) {
+          const int l = li + j * gsz;
+          xa[j] = l < k ? ya[l] : 0.0;
+          xb[j] = l < k ? yb[l] : 0.0;
+        }
+        double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+        for (int j = 0; j < kEigR; ++j) {
+          al += xa[j] * xa[j];
+          be += xb[j] * xb[j];
+          ga += xa[j] * xb[j];
+        }
+        for (int off = gsz >> 1; off > 0; off >>= 1) {
+          al += __shfl_xor(al, off, gsz);
+          be += __shfl_xor(be, off, gsz);
+          ga += __shfl_xor(ga, off, gsz);
+        }
+        if (al > 0.0 && be > 0.0) {
+          const double lim = sqrt(al) * sqrt(be);
+          if (sweep == 0 && li == 0 && fabs(ga) > tol * lim) atomicAdd(&nviol, 1);
+          if (fabs(ga) > tol_rot * lim) {
+            const double zeta = (be - al) / (2.0 * ga);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+#pragma unroll
+            for (int j = 0; j < kEigR; ++j) {
+              const int l = li + j * gsz;
+              if (l < k) {
+                ya[l] = c * xa[j] - sn * xb[j];
+                yb[l] = sn * xa[j] + c * xb[j];
+              }
+            }
+            if (li == 0) atomicAdd(cnt, 1);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const int n = *cnt;
+    const bool clean = sweep == 0 && nviol == 0;
+    if (tid == 0) nrot[(sweep + 1) & 1] = 0;
+    __syncthreads();
+    if (clean) {
+      ident = true;
+      break;
+    }
+    if (n == 0) break;
+  }
+  if (!ident) {
+    // column norms: the pairs of round 0 cover every column once
+    if (pi < npairs) {
+      const int a = circle_block(pi, 0, ke), b = circle_block(ke - 1 - pi, 0, ke);
+      double al = 0.0, be = 0.0;
+      for (int l = li; l < k; l += gsz) {
+        al += X[a * k + l] * X[a * k + l];
+        be += X[b * k + l] * X[b * k + l];
+      }
+      for (int off = gsz >> 1; off > 0; off >>= 1) {
+        al += __shfl_xor(al, off, gsz);
+        be += __shfl_xor(be, off, gsz);
+      }
+      if (li == 0) {
+        nrm[a] = sqrt(al);
+        nrm[b] = sqrt(be);
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < k * k; e += blockDim.x) {
+      const double v = nrm[e / k];
+      X[e] = v > 1e-300 ? X[e] / v : 0.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < k; ++c) {
+      if (nrm[c] > 1e-300) continue;  // uniform: nrm is shared
+      for (int r = tid; r < k; r += blockDim.x) {
+        double q = 1.0;
+        for (int j = 0; j < k; ++j) q -= X[j * k + r] * X[j * k + r];
+        vec[r] = q;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int bi = 0;
+        for (int r = 1; r < k; ++r)
+          if (vec[r] > vec[bi]) bi = r;
+        best = bi;
+      }
+      __syncthreads();
+      const int e0 = best;
+      for (int r = tid; r < k; r += blockDim.x) {
+        double q = r == e0 ? 1.0 : 0.0;
+        for (int j = 0; j < k; ++j) q -= X[j * k + r] * X[j * k + e0];
+        vec[r] = q;
+      }
+      __syncthreads();
+      for (int j = tid; j < k; j += blockDim.x) {
+        double q = 0.0;
+        for (int r = 0; r < k; ++r) q += X[j * k + r] * vec[r];
+        hh[j] = q;
+      }
+      __syncthreads();
+      for (int r = tid; r < k; r += blockDim.x) {
+        double q = vec[r];
+        for (int j = 0; j < k; ++j) q -= X[j * k + r] * hh[j];
+        vec[r] = q;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double q = 0.0;
+        for (int r = 0; r < k; ++r) q += vec[r] * vec[r];
+        nrm[c] = sqrt(q);
+      }
+      __syncthreads();
+      for (int r = tid; r < k; r += blockDim.x) X[c * k + r] = vec[r] / nrm[c];
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < k * k; e += blockDim.x) {
+    const int r = e % k, c = e / k;
+    Wg[int64_t(c) * J.ldw + r] = ident ? (r == c ? 1.0 : 0.0) : X[e];
+  }
+  if (tid == 0 && J.flag_idx >= 0) flags[J.flag_idx] = nviol;
+}
+
+void gram_eig(hipStream_t st, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
+              size_t lds_bytes) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gram_eig, dim3(n), dim3(kSmallThreads), lds_bytes, st, base,
+                     static_cast<const SmallJacobi*>(descs), flags, tol, max_sweeps);
+}
+
+size_t gram_eig_lds_bytes(int k) {
+  const int64_t ke = k + (k & 1);
+  return size_t(int64_t(k) * ke + 3 * ke) * sizeof(double);
+}
+
+size_t jacobi_small_lds_limit() {
+  static size_t limit = 0;
+  if (!limit) {
+    int dev = 0;
+    int v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    limit = size_t(v);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jacobi_small),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit) - 64);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram_eig),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit) - 64);
+  }
+  return limit;
+}
+
+// ---- finalize ----------------------------------------------------------------
+// aux = base + w_off + kp*kp holds 2*kp + ell doubles: column norms, ranks and
+// the completion vector.
+
+// One work-group per sector: singular values (column norms of Y), their rank
+// in descending order (ties broken by column index) and the sorted values.
+__global__ __launch_bounds__(256) void k_svd_norms(double* __restrict__ base, const SvdSector* __restrict__ secs) {
   const SvdSector s = secs[blockIdx.x];
   const int k = s.k, ell = s.ell, kp = s.nb * kSvdBlock;
   const double* Y = base + s.y_off;
-  const double* W = base + s.w_off;
   double* aux = base + s.w_off + int64_t(kp) * kp;
-  double* sig = aux;            // column norms
-  double* rnk = aux + kp;       // rank of each column
-  double* v = aux + 2 * kp;     // completion vector (ell)
+  double* sig = aux;
+  double* rnk = aux + kp;
   double* S = base + s.s_off;
-  const int tid = threadIdx.x;
-  // column norms: one column per thread group of 64 (wave)
-  const int wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int j = wave; j < k; j += 4) {
     double acc = 0.0;
     for (int l = lane; l < ell; l += 64) {
@@ -262,38 +596,80 @@ __global__ __launch_bounds__(256) void k_svd_finalize(double* __restrict__ base,
     else
       S[r] = sj;
   }
-  __syncthreads();
-  if (s.eig_shift) {
-    // eigenvectors: U[row, asc] = W[row, j]
-    double* U = base + s.u_off;
-    const int64_t n = int64_t(s.m) * k;
-    for (int64_t e = tid; e < n; e += blockDim.x) {
-      const int row = int(e / k), j = int(e % k);
-      const int asc = k - 1 - int(rnk[j]);
-      U[int64_t(row) * k + asc] = W[int64_t(j) * kp + row];
+}
+
+// Scatter of one 64x64 tile of a column-major source (Y: ell x k, or W: k x k
+// leading block of kp x kp) into the row-major output with permuted, scaled
+// columns, transposed through LDS so both the reads and the writes are
+// coalesced.  task = (sector, part, row0, col0); part 0: Y -> Z side
+// (columns divided by their norm, zero when numerically null), part 1: W.
+__global__ __launch_bounds__(256) void k_svd_write(double* __restrict__ base, const SvdSector* __restrict__ secs,
+                                                   const int4* __restrict__ tasks) {
+  __shared__ double tile[64][65];
+  __shared__ int perm[64];
+  __shared__ double scl[64];
+  const int4 t = tasks[blockIdx.x];
+  const SvdSector s = secs[t.x];
+  const int k = s.k, kp = s.nb * kSvdBlock;
+  const double* aux = base + s.w_off + int64_t(kp) * kp;
+  const double* sig = aux;
+  const double* rnk = aux + kp;
+  const bool zpart = t.y == 0;
+  const int rows = zpart ? s.ell : k;
+  const int64_t ld = zpart ? s.ell : kp;
+  const double* src = base + (zpart ? s.y_off : s.w_off);
+  double* dst;
+  if (s.eig_shift)
+    dst = base + s.u_off;
+  else
+    dst = base + ((zpart != (s.transposed != 0)) ? s.u_off : s.v_off);
+  const int r0 = t.z, c0 = t.w;
+  const int tid = threadIdx.x;
+  const double smax = base[s.s_off + (s.eig_shift ? k - 1 : 0)];
+  if (tid < 64) {
+    const int c = c0 + tid;
+    if (c < k) {
+      const int r = int(rnk[c]);
+      perm[tid] = s.eig_shift ? k - 1 - r : r;
+      if (zpart) {
+        const double sj = sig[c];
+        const double thr = (s.eig_shift ? 0.0 : smax) * double(s.ell) * DBL_EPSILON;
+        scl[tid] = (sj > thr && sj > 0.0) ? 1.0 / sj : 0.0;
+      } else {
+        scl[tid] = 1.0;
+      }
     }
-    return;
   }
+  for (int i = 0; i < 16; ++i) {
+    const int cl = (tid >> 6) + 4 * i;
+    const int rl = tid & 63;
+    const int r = r0 + rl, c = c0 + cl;
+    tile[cl][rl] = (r < rows && c < k) ? src[int64_t(c) * ld + r] : 0.0;
+  }
+  __syncthreads();
+  for (int i = 0; i < 16; ++i) {
+    const int rl = (tid >> 6) + 4 * i;
+    const int cl = tid & 63;
+    const int r = r0 + rl, c = c0 + cl;
+    if (r < rows && c < k) dst[int64_t(r) * k + perm[cl]] = tile[cl][rl] * scl[cl];
+  }
+}
+
+// Orthonormal completion of the Z-side columns whose singular value is
+// numerically zero (modified Gram-Schmidt, twice, against all earlier
+// columns, starting from unit vectors).  One work-group per sector.
+__global__ __launch_bounds__(256) void k_svd_complete(double* __restrict__ base,
+                                                      const SvdSector* __restrict__ secs) {
+  __shared__ double sh[256];
+  const SvdSector s = secs[blockIdx.x];
+  if (s.eig_shift) return;
+  const int k = s.k, ell = s.ell, kp = s.nb * kSvdBlock;
+  const double* S = base + s.s_off;
+  double* v = base + s.w_off + int64_t(kp) * kp + 2 * kp;
+  double* Zout = base + (s.transposed ? s.v_off : s.u_off);  // ell x k row-major
   const double smax = k > 0 ? S[0] : 0.0;
   const double null_thr = smax * double(ell) * DBL_EPSILON;
-  double* Zout = base + (s.transposed ? s.v_off : s.u_off);  // ell x k
-  double* Wout = base + (s.transposed ? s.u_off : s.v_off);  // k x k (rows = min dim)
-  {
-    const int64_t n = int64_t(ell) * k;
-    for (int64_t e = tid; e < n; e += blockDim.x) {
-      const int row = int(e / k), j = int(e % k);
-      const int r = int(rnk[j]);
-      const double sj = sig[j];
-      Zout[int64_t(row) * k + r] = (sj > null_thr && sj > 0.0) ? Y[int64_t(j) * ell + row] / sj : 0.0;
-    }
-    const int64_t nw = int64_t(k) * k;
-    for (int64_t e = tid; e < nw; e += blockDim.x) {
-      const int row = int(e / k), j = int(e % k);
-      Wout[int64_t(row) * k + int(rnk[j])] = W[int64_t(j) * kp + row];
-    }
-  }
-  __syncthreads();
-  // orthonormal completion of null columns, in rank order
+  const int tid = threadIdx.x;
   int first_null = k;
   for (int t = 0; t < k; ++t)
     if (!(S[t] > null_thr && S[t] > 0.0)) {
@@ -325,9 +701,13 @@ __global__ __launch_bounds__(256) void k_svd_finalize(double* __restrict__ base,
   }
 }
 
-void svd_finalize(hipStream_t st, double* base, const SvdSector* secs, int nsec) {
+void svd_finalize(hipStream_t st, double* base, const SvdSector* secs, const SvdSector* host_secs, int nsec,
+                  const int4* tasks, int ntasks) {
   if (nsec <= 0) return;
-  hipLaunchKernelGGL(k_svd_finalize, dim3(nsec), dim3(256), 0, st, base, secs);
+  (void)host_secs;
+  hipLaunchKernelGGL(k_svd_norms, dim3(nsec), dim3(256), 0, st, base, secs);
+  if (ntasks > 0) hipLaunchKernelGGL(k_svd_write, dim3(ntasks), dim3(256), 0, st, base, secs, tasks);
+  hipLaunchKernelGGL(k_svd_complete, dim3(nsec), dim3(256), 0, st, base, secs);
 }
 
 __global__ __launch_bounds__(256) void k_svd_shift(double* __restrict__ base, const SvdSector* __restrict__ secs) {

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <numeric>
 
@@ -81,8 +82,86 @@ struct Batched {
   int sweeps = 0;
 };
 
+void launch_gemm(Context* ctx, double* ab, const std::vector<kern::GemmOut>& outs,
+                 const std::vector<kern::GemmPair>& pairs, const char* name) {
+  std::vector<kern::GemmTile> tiles;
+  for (size_t o = 0; o < outs.size(); ++o) {
+    const int tm = (outs[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
+    const int tn = (outs[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
+    for (int i = 0; i < tm; ++i)
+      for (int j = 0; j < tn; ++j) tiles.push_back({int32_t(o), int16_t(i), int16_t(j)});
+  }
+  if (tiles.empty()) return;
+  KernelTimer kt(*ctx, name);
+  kern::gemm_f64(ctx->stream(), ab, ctx->upload_vec(tiles), int(tiles.size()), ctx->upload_vec(outs),
+                 ctx->upload_vec(pairs), 1.0, 0.0);
+}
+
+// Cyclic block Jacobi sweeps over the sectors listed in `which` until a sweep
+// rotates nothing (or max_sweeps).  first_rot[i] receives the number of
+// rotating work-groups of the first sweep of sector which[i].  Returns the
+// number of sweeps run.
+int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& secs, const std::vector<int>& which,
+                  double tol, int inner, int max_sweeps, std::vector<int>* first_rot) {
+  const int ns = int(secs.size());
+  if (which.empty()) return 0;
+  auto* d_secs = ctx->upload_vec(secs);
+  BufPtr cnt = ctx->alloc(size_t(ns) * sizeof(int32_t));
+  int32_t* d_cnt = reinterpret_cast<int32_t*>(cnt->cptr());
+  std::vector<char> active(ns, 0);
+  for (int i : which) active[i] = 1;
+  std::vector<int32_t> h_cnt(ns);
+  if (first_rot) first_rot->assign(which.size(), 0);
+  int sweeps = 0;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    std::vector<int32_t> work;
+    int max_nb = 0;
+    for (int i = 0; i < ns; ++i) {
+      if (!active[i]) continue;
+      const int nb = secs[i].nb;
+      max_nb = std::max(max_nb, nb);
+      for (int pi = 0; pi < nb / 2; ++pi) {
+        work.push_back(i);
+        work.push_back(pi);
+      }
+    }
+    if (work.empty()) break;
+    kern::fill_i32(ctx->stream(), d_cnt, ns, 0);
+    auto* d_work = ctx->upload_vec(work);
+    for (int r = 0; r < max_nb - 1; ++r) {
+      KernelTimer kt(*ctx, "svd_round");
+      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner);
+    }
+    QTT_HIP(hipMemcpyAsync(h_cnt.data(), d_cnt, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx->stream()));
+    ctx->sync();
+    sweeps = sweep + 1;
+    if (sweep == 0 && first_rot)
+      for (size_t q = 0; q < which.size(); ++q) (*first_rot)[q] = h_cnt[which[q]];
+    bool any = false;
+    for (int i = 0; i < ns; ++i) {
+      if (active[i] && h_cnt[i] == 0) active[i] = 0;
+      any |= bool(active[i]);
+    }
+    if (!any) break;
+  }
+  return sweeps;
+}
+
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : dflt;
+}
+
 // Gathers the sector matrices of a rank-2 tensor and runs the batched Jacobi
 // SVD (eigh_mode: shifted symmetric eigen-decomposition) on device.
+//
+// Sectors much longer than wide (ell >= 2 kp) are first preconditioned by
+// Gram iterations: G = Y^T Y (kp x kp, MFMA GEMM), its eigenvectors W_g by the
+// same block Jacobi on G, Y <- Y W_g and W <- W W_g, repeated until the
+// Jacobi on G rotates nothing in its first sweep, i.e. the columns of Y are
+// orthogonal to the tolerance.  Every step is an exact orthogonal transform
+// of Y, so the singular values still come from the column norms of Y.
 Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool eigh_mode) {
   Context* ctx = t.ctx;
   double* ab = arena_base(ctx);
@@ -90,7 +169,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   const int ns = int(plans.size());
   B.secs.resize(ns);
   int64_t work_elems = 0, out_elems = 0;
-  std::vector<int64_t> yo(ns), wo(ns);
+  std::vector<int64_t> yo(ns), wo(ns), wsz(ns);
   B.u_off.resize(ns);
   B.v_off.resize(ns);
   B.s_off.resize(ns);
@@ -111,7 +190,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     yo[i] = work_elems;
     work_elems += int64_t(s.ell) * kp;
     wo[i] = work_elems;
-    work_elems += kp * kp + 2 * kp + s.ell + 1;  // W + finalize scratch + eigh shift
+    wsz[i] = kp * kp + 2 * kp + s.ell + 1;  // W + finalize scratch + eigh shift
+    work_elems += wsz[i];
     B.u_off[i] = out_elems;
     out_elems += int64_t(s.m) * s.k;
     B.v_off[i] = out_elems;
@@ -187,54 +267,231 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
   }
   if (ns == 0) return B;
-  auto* d_secs = ctx->upload_vec(B.secs);
   {
+    auto* d_secs = ctx->upload_vec(B.secs);
     KernelTimer kt(*ctx, "svd_init");
     kern::svd_init_w(ctx->stream(), ab, d_secs, ns);
     if (eigh_mode) kern::svd_shift(ctx->stream(), ab, d_secs, ns);
   }
-  BufPtr cnt = ctx->alloc(size_t(ns) * sizeof(int32_t));
-  int32_t* d_cnt = reinterpret_cast<int32_t*>(cnt->cptr());
-  std::vector<char> active(ns, 1);
-  std::vector<int32_t> h_cnt(ns);
   int max_ell = 0;
   for (auto& s : B.secs) max_ell = std::max(max_ell, s.ell);
   const double tol = 8.0 * std::sqrt(double(std::max(max_ell, 16))) * DBL_EPSILON;
-  const int max_sweeps = 20;
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    std::vector<int32_t> work;
-    int max_nb = 0;
-    for (int i = 0; i < ns; ++i) {
-      if (!active[i]) continue;
-      const int nb = B.secs[i].nb;
-      max_nb = std::max(max_nb, nb);
-      for (int pi = 0; pi < nb / 2; ++pi) {
-        work.push_back(i);
-        work.push_back(pi);
+  const int inner = env_int("QTT_SVD_INNER_SWEEPS", 1);
+  const int max_sweeps = 30;
+  const size_t lds_limit = kern::jacobi_small_lds_limit();
+  const size_t lds_margin = 256;
+  auto lds_bytes = [](int64_t ell, int64_t k) { return size_t(ell * (k + (k & 1))) * sizeof(double); };
+  const bool use_small = env_int("QTT_SVD_SMALL", 1) != 0;
+  const bool gram_multi = env_int("QTT_SVD_GRAM_SOLVER", 0) == 1;
+  std::vector<char> done(ns, 0), big_only(ns, 0);
+  std::vector<BufPtr> keep;  // Gram and ping-pong buffers live until the finalize is enqueued
+
+  // ---- Gram preconditioning of long sectors ---------------------------------
+  std::vector<int> wide;
+  for (int i = 0; i < ns; ++i) {
+    const auto& s = B.secs[i];
+    const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+    const bool long_enough = s.ell >= 2 * kp || env_int("QTT_SVD_GRAM_ALL", 1) != 0;
+    if (!eigh_mode && use_small && long_enough && s.k <= kern::kGramEigMaxK &&
+        (gram_multi || kern::gram_eig_lds_bytes(s.k) + lds_margin <= lds_limit))
+      wide.push_back(i);
+  }
+  if (!wide.empty()) {
+    const int nw = int(wide.size());
+    int64_t elems = 0;
+    std::vector<int64_t> go(nw), gwo(nw), y2o(nw), w2o(nw);
+    size_t lds_need = 0;
+    for (int q = 0; q < nw; ++q) {
+      const auto& s = B.secs[wide[q]];
+      const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+      go[q] = elems;
+      elems += int64_t(s.k) * (gram_multi ? kp : s.k);
+      gwo[q] = elems;
+      elems += gram_multi ? kp * kp : int64_t(s.k) * s.k;
+      y2o[q] = elems;
+      elems += int64_t(s.ell) * kp;
+      w2o[q] = elems;
+      elems += wsz[wide[q]];
+      lds_need = std::max(lds_need, kern::gram_eig_lds_bytes(s.k));
+    }
+    BufPtr gbuf = ctx->alloc(size_t(elems) * sizeof(double));
+    keep.push_back(gbuf);
+    const int64_t gb = gbuf->dptr() - ab;
+    {  // ping-pong partners: Y2 zero (padding columns stay zero), W2 identity
+      std::vector<kern::Copy2D> z;
+      std::vector<kern::ChunkTask> zt;
+      std::vector<kern::SvdSector> wi;
+      for (int q = 0; q < nw; ++q) {
+        const auto& s = B.secs[wide[q]];
+        const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+        if (gram_multi) {  // padding columns of G
+          kern::Copy2D c{};
+          c.src_off = -1;
+          c.dst_off = gb + go[q] + int64_t(s.k) * s.k;
+          c.rows = 1;
+          c.cols = int32_t(int64_t(s.k) * (kp - s.k));
+          c.dst_cs = 1;
+          add_chunks(zt, int(z.size()), int64_t(c.cols));
+          z.push_back(c);
+        }
+        kern::Copy2D c{};
+        c.src_off = -1;
+        c.dst_off = gb + y2o[q];
+        c.rows = 1;
+        c.cols = int32_t(int64_t(s.ell) * kp);
+        c.dst_cs = 1;
+        add_chunks(zt, int(z.size()), int64_t(c.cols));
+        z.push_back(c);
+        kern::SvdSector w = s;
+        w.w_off = gb + w2o[q];
+        wi.push_back(w);
       }
+      KernelTimer kt(*ctx, "svd_init");
+      kern::copy2d(ctx->stream(), ab, ctx->upload_vec(z), ctx->upload_vec(zt), int(zt.size()));
+      kern::svd_init_w(ctx->stream(), ab, ctx->upload_vec(wi), nw);
     }
-    if (work.empty()) break;
-    kern::fill_i32(ctx->stream(), d_cnt, ns, 0);
-    auto* d_work = ctx->upload_vec(work);
-    for (int r = 0; r < max_nb - 1; ++r) {
-      KernelTimer kt(*ctx, "svd_round");
-      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol);
+    std::vector<int64_t> alt_y(nw), alt_w(nw);
+    for (int q = 0; q < nw; ++q) {
+      alt_y[q] = gb + y2o[q];
+      alt_w[q] = gb + w2o[q];
     }
-    QTT_HIP(hipMemcpyAsync(h_cnt.data(), d_cnt, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
-                           ctx->stream()));
-    ctx->sync();
-    B.sweeps = sweep + 1;
-    bool any = false;
-    for (int i = 0; i < ns; ++i) {
-      if (active[i] && h_cnt[i] == 0) active[i] = 0;
-      any |= bool(active[i]);
+    BufPtr fbuf = ctx->alloc(size_t(nw) * sizeof(int32_t));
+    keep.push_back(fbuf);
+    int32_t* d_flags = reinterpret_cast<int32_t*>(fbuf->cptr());
+    std::vector<int32_t> h_flags(nw);
+    std::vector<int> act(nw);
+    std::iota(act.begin(), act.end(), 0);
+    for (int it = 0; it < 6 && !act.empty(); ++it) {
+      // G = Y^T Y over the k real columns; its eigenvectors W_g in one work-group
+      std::vector<kern::GemmOut> outs;
+      std::vector<kern::GemmPair> prs;
+      std::vector<kern::SmallJacobi> jd;
+      for (int q : act) {
+        const auto& s = B.secs[wide[q]];
+        kern::GemmPair gp{};
+        gp.a_off = s.y_off;
+        gp.a_rs = s.ell;
+        gp.a_cs = 1;
+        gp.b_off = s.y_off;
+        gp.b_rs = 1;
+        gp.b_cs = s.ell;
+        gp.K = s.ell;
+        outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
+        prs.push_back(gp);
+        jd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, q, 0});
+      }
+      launch_gemm(ctx, ab, outs, prs, "svd_gram");
+      if (gram_multi) {
+        std::vector<kern::SvdSector> gsec;
+        std::vector<int> all;
+        for (int q : act) {
+          kern::SvdSector g = B.secs[wide[q]];
+          g.y_off = gb + go[q];
+          g.w_off = gb + gwo[q];
+          g.ell = g.k;
+          g.flag_idx = int32_t(gsec.size());
+          all.push_back(int(gsec.size()));
+          gsec.push_back(g);
+        }
+        {
+          KernelTimer kt(*ctx, "svd_init");
+          kern::svd_init_w(ctx->stream(), ab, ctx->upload_vec(gsec), int(gsec.size()));
+        }
+        std::vector<int> fr;
+        jacobi_sweeps(ctx, ab, gsec, all, tol, inner, max_sweeps, &fr);
+        for (size_t j = 0; j < act.size(); ++j) h_flags[act[j]] = fr[j];
+      } else {
+        KernelTimer kt(*ctx, "svd_eig");
+        kern::gram_eig(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), d_flags, tol, max_sweeps, lds_need);
+      }
+      if (it >= 1) {  // G diagonal to tol: the columns of Y are orthogonal, the sector is done
+        if (!gram_multi) {
+          QTT_HIP(hipMemcpyAsync(h_flags.data(), d_flags, size_t(nw) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 ctx->stream()));
+          ctx->sync();
+        }
+        std::vector<int> next;
+        for (int q : act) {
+          if (h_flags[q] == 0)
+            done[wide[q]] = 1;
+          else
+            next.push_back(q);
+        }
+        act.swap(next);
+        if (act.empty()) break;
+      }
+      // Y2^T = W_g^T Y^T and W2^T = W_g^T W^T (leading k x k block of W)
+      outs.clear();
+      prs.clear();
+      for (int q : act) {
+        const auto& s = B.secs[wide[q]];
+        const int kp = s.nb * kern::kSvdBlock;
+        kern::GemmPair p1{};
+        p1.a_off = gb + gwo[q];
+        p1.a_rs = gram_multi ? kp : s.k;
+        p1.a_cs = 1;
+        p1.b_off = s.y_off;
+        p1.b_rs = s.ell;
+        p1.b_cs = 1;
+        p1.K = s.k;
+        outs.push_back({alt_y[q], s.k, s.ell, int32_t(prs.size()), int32_t(prs.size() + 1), s.ell, 0});
+        prs.push_back(p1);
+        kern::GemmPair p2 = p1;
+        p2.b_off = s.w_off;
+        p2.b_rs = kp;
+        outs.push_back({alt_w[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), kp, 0});
+        prs.push_back(p2);
+      }
+      launch_gemm(ctx, ab, outs, prs, "svd_rotate");
+      for (int q : act) {
+        auto& s = B.secs[wide[q]];
+        std::swap(s.y_off, alt_y[q]);
+        std::swap(s.w_off, alt_w[q]);
+      }
+      B.sweeps += 1;
     }
-    if (!any) break;
+    for (int q : act) big_only[wide[q]] = 1;  // keeps its accumulated W: multi-WG path only
+  }
+  // ---- Jacobi on everything not yet orthogonal --------------------------------
+  std::vector<int> rest_small, rest_big;
+  size_t small_need = 0;
+  for (int i = 0; i < ns; ++i) {
+    if (done[i]) continue;
+    const auto& s = B.secs[i];
+    if (use_small && !big_only[i] && lds_bytes(s.ell, s.k) + lds_margin <= lds_limit) {
+      rest_small.push_back(i);
+      small_need = std::max(small_need, lds_bytes(s.ell, s.k));
+    } else {
+      rest_big.push_back(i);
+    }
+  }
+  if (!rest_small.empty()) {
+    std::vector<kern::SmallJacobi> jd;
+    for (int i : rest_small) {
+      const auto& s = B.secs[i];
+      jd.push_back({s.y_off, s.w_off, s.ell, s.k, s.ell, s.nb * kern::kSvdBlock, -1, 0});
+    }
+    KernelTimer kt(*ctx, "svd_small");
+    kern::jacobi_small(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), nullptr, tol, max_sweeps,
+                       small_need);
+  }
+  B.sweeps += jacobi_sweeps(ctx, ab, B.secs, rest_big, tol, inner, max_sweeps, nullptr);
+  // ---- finalize --------------------------------------------------------------
+  std::vector<int4> wt;
+  for (int i = 0; i < ns; ++i) {
+    const auto& s = B.secs[i];
+    if (!eigh_mode)
+      for (int r = 0; r < s.ell; r += 64)
+        for (int c = 0; c < s.k; c += 64) wt.push_back(make_int4(i, 0, r, c));
+    for (int r = 0; r < s.k; r += 64)
+      for (int c = 0; c < s.k; c += 64) wt.push_back(make_int4(i, 1, r, c));
   }
   {
     KernelTimer kt(*ctx, "svd_finalize");
-    kern::svd_finalize(ctx->stream(), ab, d_secs, ns);
+    kern::svd_finalize(ctx->stream(), ab, ctx->upload_vec(B.secs), B.secs.data(), ns, ctx->upload_vec(wt),
+                       int(wt.size()));
   }
+  // the work buffers are released after the finalize kernels in stream order
   return B;
 }
 

@@ -299,6 +299,7 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     gouts[o].c_off = out.ptr(cb_) - ab;
     gouts[o].M = int32_t(M);
     gouts[o].N = int32_t(N);
+    gouts[o].ldc = int32_t(N);
     gouts[o].pair_begin = int32_t(gpairs.size());
     int64_t ksum = 0;
     for (auto& p : outs[o].pairs) {

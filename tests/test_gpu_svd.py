@@ -106,6 +106,51 @@ def test_truncated_svd_structure_exact(ctx, tol, mn, mx):
         assert np.abs(d.blocks[k] - rd.blocks[k]).max() <= 1e-12
 
 
+def test_svd_wide_sectors_gram_path(ctx):
+    """Sectors much longer than wide go through the Gram preconditioning;
+    both orientations (m << n and m >> n) and a graded spectrum."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(51)
+    rows = [(20, (1,)), (30, (-1,)), (5, (3,))]
+    cols = [(100, (-1,)), (90, (-1,)), (120, (1,)), (70, (1,)), (40, (-3,))]
+    for legs in ([rows, cols], [cols, rows]):
+        A = rand_btensor(rng, G1, legs, (0,))
+        # graded singular values down to 1e-12 relative
+        for k in A.blocks:
+            b = A.blocks[k]
+            A.blocks[k] = b * np.logspace(0, -12, b.shape[1])[None, :] if b.shape[1] < b.shape[0] else \
+                b * np.logspace(0, -12, b.shape[0])[:, None]
+        rU, rd, rV = svd_split(A, 1)
+        U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, A), 1))
+        assert_same_structure(d, rd)
+        assert_same_structure(U, rU)
+        smax_all = max(float(b.max()) for b in rd.blocks.values())
+        for k in rd.blocks:
+            assert np.abs(d.blocks[k] - rd.blocks[k]).max() <= 1e-12 * smax_all
+        _check_svd(A, U, d, V, 1)
+
+
+def test_svd_contract_svd_workload_small(ctx):
+    """The benchmark's contraction + SVD at D=64 against the oracle."""
+    import quantit_amd as qa
+    from oracle.btensor_oracle import BTensor as OB
+    from quantit_amd import workloads
+    X, A, dx, da, split = workloads.contract_svd(64)
+    Xo, Ao = OB(X["groups"], X["secs"], X["cvals"], X["sel"], X["blocks"]), OB(A["groups"], A["secs"], A["cvals"],
+                                                                                A["sel"], A["blocks"])
+    Y = tensordot(Xo, Ao, dx, da)
+    rU, rd, rV = svd_truncated(Y, split, 1e-10)
+    y = qa.BTensor.from_parts(ctx, **X).tensordot(qa.BTensor.from_parts(ctx, **A), dx, da)
+    U, d, V = (from_dev(x) for x in qa.svd(y, split, tol=1e-10))
+    assert_same_structure(d, rd)
+    assert_same_structure(U, rU)
+    assert_same_structure(V, rV)
+    smax_all = max(float(b.max()) for b in rd.blocks.values())
+    for k in rd.blocks:
+        assert np.abs(d.blocks[k] - rd.blocks[k]).max() <= 1e-12 * smax_all
+    _check_svd(Y, U, d, V, split)
+
+
 def test_svd_rank_deficient_orthonormal(ctx):
     """Sectors with exactly zero singular values: U and V stay orthonormal."""
     import quantit_amd as qa
