@@ -77,6 +77,34 @@ def cpu_baseline(X, A, dx, da, split, budget_s=20.0):
     return float(np.median(times)), len(times), threads
 
 
+def copy_probe(mib):
+    """One plain device-to-device copy of a known size: the scale for the
+    FETCH_SIZE / WRITE_SIZE counters (expected mib*1024 KiB each)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes": n * 8}))
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (lines 'traffic <kernel> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_contract_svd_D1024.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,10 +114,16 @@ def main():
     ap.add_argument("--per-block", action="store_true", help="one GEMM launch per block pair (baseline form)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-split", action="store_true", help="skip the split-FP32 side measurement")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import quantit_amd as qa
     from quantit_amd import workloads
@@ -134,14 +168,14 @@ def main():
     if dominant.startswith("gemm"):
         achieved = per_launch_gemm / max(gemm_avg_s, 1e-12) / 1e12
         roof = {"kernel": gemm_name, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None}
+                "frac": achieved / peak, "traffic": read_traffic(gemm_name)}
     else:
         d_ms, d_n = kt[dominant]
         # block SVD: model FLOPs of one step spread over its launches of this kernel
         per_launch = svd_flops * args.steps / max(d_n, 1)
         achieved = per_launch / max((d_ms / max(d_n, 1)) * 1e-3, 1e-12) / 1e12
         roof = {"kernel": dominant, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": read_traffic(dominant),
                 "note": "block-SVD model FLOPs (14mn^2+8n^3) per launch over the average launch time"}
 
     out = {
@@ -167,6 +201,30 @@ def main():
         "kernel_launches_per_step": {k: v[1] / args.steps for k, v in kt.items()},
         "roofline": roof,
     }
+    if not args.no_split and args.precision == "fp64" and not args.per_block:
+        # side measurement: the same step with hi/lo split-FP32 MFMA contractions
+        y64 = x.tensordot(a, dx, da).to_parts()[4]
+        ctx.set_precision(qa.PRECISION_SPLIT_FP32)
+        ysp = x.tensordot(a, dx, da).to_parts()[4]
+        err = max(float(np.abs(ysp[k] - y64[k]).max()) for k in y64) / max(float(np.abs(v).max()) for v in y64.values())
+        for _ in range(2):
+            step()
+        ctx.sync()
+        ctx.kernel_times(reset=True)
+        ctx.set_profiling(True)
+        n_sp = max(args.steps // 2, 1)
+        t0 = time.perf_counter()
+        for _ in range(n_sp):
+            step()
+        ctx.sync()
+        t_sp = (time.perf_counter() - t0) * 1e3 / n_sp
+        ctx.set_profiling(False)
+        kts = ctx.kernel_times(reset=True)
+        ctx.set_precision(qa.PRECISION_FP64)
+        gs_ms = kts.get("gemm_split", (0.0, 0))[0] / n_sp
+        out["split_fp32"] = {"ms_per_step": t_sp, "contraction_ms": gs_ms,
+                             "contraction_tflops": contract_flops / (gs_ms * 1e-3) / 1e12 if gs_ms else None,
+                             "contraction_max_abs_err_rel_to_max": err}
     if not args.no_cpu_baseline:
         t_cpu, n_cpu, threads = cpu_baseline(X, A, dx, da, split, args.cpu_budget)
         out["cpu_baseline"] = {"value": flops / t_cpu / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
