@@ -105,6 +105,150 @@ def read_traffic(kernel):
     return None
 
 
+# Whole-solve workloads: (L, bonds, max bond, sweeps, seed, description)
+DMRG_WORKLOADS = {
+    "chain_L32_D128": dict(L=32, geometry="chain", width=0, max_bond=128, sweeps=6, seed=0,
+                           desc="Heisenberg chain J=1 OBC L=32, U(1) Sz=0, two-site DMRG, D<=128, cutoff 1e-10, "
+                                "6 sweeps, random bMPS seed 0"),
+    "cylinder4x16_D1024": dict(L=64, geometry="cylinder", width=4, max_bond=1024, sweeps=6, seed=1,
+                               desc="Heisenberg 4x16 cylinder (periodic width 4, open length 16, snake order) J=1, "
+                                    "U(1) Sz=0, two-site DMRG to D=1024, cutoff 1e-10, random bMPS seed 1"),
+}
+
+
+def _dmrg_inputs(qa, ctx, w):
+    from quantit_amd import models
+    L = w["L"]
+    bonds = models.chain_bonds(L) if w["geometry"] == "chain" else models.cylinder_bonds(w["width"], L // w["width"])
+    mpo_parts = models.bmpo_parts(models.spin_mpo(L, bonds))
+    mps_parts = models.random_bmps_parts(L, 4, q=0, seed=w["seed"])
+    return mpo_parts, mps_parts
+
+
+def bench_dmrg(args):
+    """One DMRG solve of a whole-solve workload: seconds per sweep (timed by
+    the library around each sweep), energies, FLOPs from the plans."""
+    import quantit_amd as qa
+    w = DMRG_WORKLOADS[args.workload]
+    sweeps = args.sweeps or w["sweeps"]
+    ctx = qa.Context(0, arena_bytes=64 << 30,
+                     precision=qa.PRECISION_SPLIT_FP32 if args.precision == "split" else qa.PRECISION_FP64)
+    ctx.set_per_block(args.per_block)
+    mpo_parts, mps_parts = _dmrg_inputs(qa, ctx, w)
+    opts = qa.DmrgOptions(cutoff=1e-10, convergence_criterion=0.0, maximum_bond=w["max_bond"], minimum_bond=4,
+                          maximum_iterations=sweeps)
+
+    def solve(profile):
+        mpo = [qa.BTensor.from_parts(ctx, **p) for p in mpo_parts]
+        mps = qa.mps_prepare([qa.BTensor.from_parts(ctx, **p) for p in mps_parts])
+        ctx.sync()
+        ctx.kernel_times(reset=True)
+        ctx.set_profiling(profile)
+        t0 = time.perf_counter()
+        E, mps, log = qa.dmrg(mpo, mps, opts)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        ctx.set_profiling(False)
+        return E, log, wall, ctx.kernel_times(reset=True)
+
+    # warmup: a short solve of the same workload (first launches, arena growth)
+    if args.warmup:
+        wopts = opts
+        opts = qa.DmrgOptions(cutoff=1e-10, convergence_criterion=0.0, maximum_bond=min(w["max_bond"], 32),
+                              minimum_bond=4, maximum_iterations=1)
+        solve(False)
+        opts = wopts
+    E, log, wall, _ = solve(False)
+    kt = {}
+    if w["max_bond"] <= 256:   # per-kernel breakdown from a second, profiled solve
+        _, _, _, kt = solve(True)
+    ms = log["sweep_ms"]
+    flops = log["contract_flops"] + log["svd_flops"]
+    out = {
+        "metric": "DMRG sweep s & contraction TFLOP/s @|dE/E|<=1e-10; served solves/s + p99 latency",
+        "value": float(np.mean(ms)) / 1e3,
+        "unit": "s/sweep",
+        "n_gpus": 1,
+        "steps": len(ms),
+        "warmup": args.warmup,
+        "ms_per_step": float(np.mean(ms)),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 (split-fp32 contraction)",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {w['desc']}", "precision": args.precision,
+                   "per_block_launch": args.per_block},
+        "energy": E,
+        "sweep_energies": log["energies"],
+        "sweep_ms": ms,
+        "max_bond": log["max_bond"],
+        "solve_wall_s": wall,
+        "solve_tflops": flops / wall / 1e12,
+        "contract_gflop": log["contract_flops"] / 1e9,
+        "svd_model_gflop": log["svd_flops"] / 1e9,
+        "kernel_ms_per_solve": {k: v[0] for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0])[:12]},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    if kt:
+        dom, (d_ms, d_n) = max(kt.items(), key=lambda kv: kv[1][0])
+        if dom.startswith("gemm"):
+            per = log["contract_flops"] / max(d_n, 1)
+        else:
+            per = log["svd_flops"] / max(d_n, 1)
+        ach = per / ((d_ms / max(d_n, 1)) * 1e-3) / 1e12
+        peak = PEAK_FP32_TFLOPS if dom == "gemm_split" else PEAK_FP64_TFLOPS
+        out["roofline"] = {"kernel": dom, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                           "frac": ach / peak, "traffic": None,
+                           "note": "model FLOPs of the solve attributed to the kernel's launches"}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = dmrg_cpu_baseline(mpo_parts, mps_parts, opts, args.cpu_budget, E, log)
+    print(json.dumps(out))
+
+
+def dmrg_cpu_baseline(mpo_parts, mps_parts, opts, budget, E_dev, log_dev):
+    """The oracle's DMRG (numpy restatement of sources/dmrg.cpp) on the same
+    MPO and starting state, sweep by sweep until the budget is spent."""
+    from oracle import dmrg_oracle as dor
+    from oracle.btensor_oracle import BTensor as OB
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([p.get("num_threads", 1) for p in threadpool_info()] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    mpo = [OB(**p) for p in mpo_parts]
+    mps = dor.prepare_state([OB(**p) for p in mps_parts])
+    times, energies = [], []
+    t_start = time.perf_counter()
+    last = [t_start]
+
+    class Budget(Exception):
+        pass
+
+    def on_sweep(it, E, m):
+        now = time.perf_counter()
+        times.append(now - last[0])
+        energies.append(E)
+        last[0] = now
+        if now - t_start > budget:
+            raise Budget()
+
+    o = dor.DmrgOptions(cutoff=opts.cutoff, convergence_criterion=opts.convergence_criterion,
+                        maximum_bond=opts.maximum_bond, minimum_bond=opts.minimum_bond,
+                        maximum_iterations=opts.maximum_iterations)
+    try:
+        dor.dmrg(mpo, mps, o, log=on_sweep)
+    except Budget:
+        pass
+    n = len(times)
+    dev_e = log_dev["energies"][:n]
+    dev = [abs(a - b) / abs(b) for a, b in zip(dev_e, energies)]
+    return {"value": float(np.mean(times)), "unit": "s/sweep", "cores": threads, "kind": "port",
+            "sample": f"first {n} sweeps of the same solve (numpy oracle, same MPO and start state)",
+            "sweep_s": times, "energies": energies, "max_rel_dE_vs_device": max(dev) if dev else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,12 +261,18 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the split-FP32 side measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", choices=["contract_svd_D1024"] + sorted(DMRG_WORKLOADS),
+                    default="contract_svd_D1024")
+    ap.add_argument("--sweeps", type=int, default=0, help="DMRG workloads: sweeps (0 = the workload's own)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload in DMRG_WORKLOADS:
+        bench_dmrg(args)
         return
 
     import quantit_amd as qa

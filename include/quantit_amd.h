@@ -127,6 +127,13 @@ typedef struct qtt_dmrg_options {
 int qtt_dmrg(int L, qtt_btensor* const* mpo, qtt_btensor** mps, const qtt_dmrg_options* options, double* energy,
              int max_log, double* sweep_energies, double* sweep_ms, int64_t* max_bond, int* n_sweeps,
              double* contract_flops, double* svd_flops);
+/* bMPS::move_oc (sources/MPT.cpp:74-111): moves the orthogonality centre of
+ * the L tensors mps[] from oc to target with untruncated block SVDs; the
+ * handles are replaced in place. */
+int qtt_mps_move_oc(int L, qtt_btensor** mps, int oc, int target);
+/* Preparation of a random bMPS before dmrg (tests/DMRG_heisenberg_test.h:34-37):
+ * mps[0] /= sqrt(<psi|psi>), move_oc(L-1), move_oc(0).  Handles replaced. */
+int qtt_mps_prepare(int L, qtt_btensor** mps);
 /* contract(bMPS, bMPS) and contract(bMPS, bMPS, bMPO) (sources/MPT.cpp:211-292) */
 int qtt_contract_norm(int L, qtt_btensor* const* a, qtt_btensor* const* b, double* out);
 int qtt_contract_energy(int L, qtt_btensor* const* a, qtt_btensor* const* b, qtt_btensor* const* mpo, double* out);

@@ -99,6 +99,8 @@ SIGNATURES = {
     "qtt_truncate": (C.c_int, [P, P, P, C.c_size_t, C.c_size_t, C.c_double, C.c_double, PP, PP, PP]),
     "qtt_dmrg": (C.c_int, [C.c_int, PP, PP, C.POINTER(DmrgOptionsC), DP, C.c_int, DP, DP, I64P, C.POINTER(C.c_int),
                            DP, DP]),
+    "qtt_mps_move_oc": (C.c_int, [C.c_int, PP, C.c_int, C.c_int]),
+    "qtt_mps_prepare": (C.c_int, [C.c_int, PP]),
     "qtt_contract_norm": (C.c_int, [C.c_int, PP, PP, DP]),
     "qtt_contract_energy": (C.c_int, [C.c_int, PP, PP, PP, DP]),
 }
@@ -388,6 +390,31 @@ def dmrg(mpo, mps, options):
     log = {"energies": es[:n].tolist(), "sweep_ms": ms[:n].tolist(), "max_bond": mb[:n].tolist(),
            "contract_flops": cf.value, "svd_flops": sf.value}
     return E.value, new_mps, log
+
+
+def _replace_mps(fn, mps, *args):
+    """Calls fn(L, handles, *args); the library replaces the handles in place
+    on success (the old ones are destroyed) and leaves them on failure."""
+    L = len(mps)
+    ctx = mps[0].ctx
+    hs = (C.c_void_p * L)(*[t.h for t in mps])
+    st = fn(L, C.cast(hs, PP), *args)
+    _check(st)
+    for t in mps:
+        t.h = None
+    return [BTensor(ctx, C.c_void_p(hs[i])) for i in range(L)]
+
+
+def mps_move_oc(mps, oc, target):
+    """``bMPS::move_oc`` (sources/MPT.cpp:74-111) on device; returns the new list."""
+    return _replace_mps(library().qtt_mps_move_oc, mps, int(oc), int(target))
+
+
+def mps_prepare(mps):
+    """Normalise a bMPS and leave it right-canonical with the orthogonality
+    centre at 0, as the reference's drivers do before ``dmrg``
+    (tests/DMRG_heisenberg_test.h:34-37); returns the new list."""
+    return _replace_mps(library().qtt_mps_prepare, mps)
 
 
 def contract_norm(a, b):

@@ -406,4 +406,36 @@ int qtt_contract_energy(int L, qtt_btensor* const* a, qtt_btensor* const* b, qtt
   });
 }
 
+int qtt_mps_move_oc(int L, qtt_btensor** mps, int oc, int target) {
+  return guard([&] {
+    need(mps, "mps");
+    std::vector<qtt::BTensor> v;
+    for (int i = 0; i < L; ++i) {
+      need(mps[i], "mps[i]");
+      v.push_back(mps[i]->t);
+    }
+    qtt::mps_move_oc(v, oc, target);
+    for (int i = 0; i < L; ++i) {
+      delete mps[i];
+      mps[i] = wrap(std::move(v[i]));
+    }
+  });
+}
+
+int qtt_mps_prepare(int L, qtt_btensor** mps) {
+  return guard([&] {
+    need(mps, "mps");
+    std::vector<qtt::BTensor> v;
+    for (int i = 0; i < L; ++i) {
+      need(mps[i], "mps[i]");
+      v.push_back(mps[i]->t);
+    }
+    qtt::mps_prepare(v);
+    for (int i = 0; i < L; ++i) {
+      delete mps[i];
+      mps[i] = wrap(std::move(v[i]));
+    }
+  });
+}
+
 }  // extern "C"

@@ -240,4 +240,33 @@ double contract_energy(const std::vector<BTensor>& a, const std::vector<BTensor>
   return item(tensordot(left, right, {0, 1, 2}, {0, 1, 2}));
 }
 
+void mps_move_oc(std::vector<BTensor>& mps, int& oc, int target) {
+  const int L = int(mps.size());
+  if (target < 0 || target >= L) throw InvalidArgument("Proposed orthogonality center falls outside the MPS");
+  if (oc < 0 || oc >= L) throw InvalidArgument("orthogonality center outside the MPS");
+  while (target < oc) {
+    SvdResult r = svd(mps[oc], 1);
+    mps[oc] = permute(conj(r.V), {2, 0, 1});
+    mps[oc - 1] = tensordot(mps[oc - 1], mul(r.U, r.d), {2}, {0});
+    --oc;
+  }
+  while (target > oc) {
+    SvdResult r = svd(mps[oc], 2);
+    mps[oc] = std::move(r.U);
+    mps[oc + 1] = tensordot(conj(mul(r.V, r.d)), mps[oc + 1], {0}, {0});
+    ++oc;
+  }
+}
+
+void mps_prepare(std::vector<BTensor>& mps) {
+  const int L = int(mps.size());
+  if (L < 1) throw InvalidArgument("empty MPS");
+  const double n2 = contract_norm(mps, mps);
+  if (!(n2 > 0.0)) throw DomainError("MPS of zero norm");
+  mps[0] = scale(mps[0], 1.0 / std::sqrt(n2));
+  int oc = 0;
+  mps_move_oc(mps, oc, L - 1);
+  mps_move_oc(mps, oc, 0);
+}
+
 }  // namespace qtt

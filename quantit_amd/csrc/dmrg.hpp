@@ -53,6 +53,14 @@ BTensor compute_left_env(const BTensor& W, const BTensor& Y, const BTensor& L);
 BTensor compute_right_env(const BTensor& W, const BTensor& Y, const BTensor& R);
 BTensor hamil2site_times_state(const BTensor& theta, const BTensor& W12, const BTensor& L, const BTensor& R);
 
+// bMPS::move_oc (sources/MPT.cpp:74-111): moves the orthogonality centre from
+// oc to target with untruncated block SVDs; oc is updated.
+void mps_move_oc(std::vector<BTensor>& mps, int& oc, int target);
+// The preparation the reference's DMRG drivers apply to a random bMPS
+// (tests/DMRG_heisenberg_test.h:34-37): A_0 /= sqrt(<psi|psi>), then the
+// orthogonality centre goes to L-1 and back to 0.
+void mps_prepare(std::vector<BTensor>& mps);
+
 // <a|b> and <a|H|b> (sources/MPT.cpp:211-292) for checks.
 double contract_norm(const std::vector<BTensor>& a, const std::vector<BTensor>& b);
 double contract_energy(const std::vector<BTensor>& a, const std::vector<BTensor>& b, const std::vector<BTensor>& mpo);

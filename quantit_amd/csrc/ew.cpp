@@ -44,7 +44,8 @@ BTensor mul(const BTensor& x_in, const BTensor& y_in) {
     const int dl = lr - n, ds = sr - n;
     const bool s_one = y.shape.nsec(ds) == 1 && y.shape.secs[ds][0] == 1;
     const bool l_one = x.shape.nsec(dl) == 1 && x.shape.secs[dl][0] == 1;
-    if (s_one || l_one) throw InvalidArgument("broadcast over singleton dimensions is not supported");
+    // both singleton: the sections coincide and the product is elementwise
+    if (s_one != l_one) throw InvalidArgument("broadcast over singleton dimensions is not supported");
     if (x.shape.nsec(dl) != y.shape.nsec(ds))
       throw InvalidArgument("The section and tensor size of tensor a must match the section size and tensor size of "
                             "tensor b at non-singleton dimension " +

@@ -138,11 +138,12 @@ void jacobi_small(hipStream_t s, double* base, const void* descs, int n, int* fl
                   size_t lds_bytes);
 size_t jacobi_small_lds_limit();
 // Eigenvectors of k x k Gram matrices (SmallJacobi with ell = k): W written
-// k x k at w_off with leading dimension ldw; flags[flag_idx] = rotations of
-// the first sweep (0: W = I).
-constexpr int kGramEigMaxK = 144;
+// k x k at w_off with leading dimension ldw; flags[flag_idx] = pairs of G
+// failing the orthogonality test (0: W = I).  precond: pivoted-Cholesky
+// preconditioning.
+constexpr int kGramEigMaxK = 140;
 void gram_eig(hipStream_t s, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
-              size_t lds_bytes);
+              int precond, size_t lds_bytes);
 size_t gram_eig_lds_bytes(int k);  // largest dynamic LDS per work-group (bytes)
 
 // Global truncation (sources/btensor_linalg.cpp:657-755,

@@ -343,29 +343,37 @@ void jacobi_small(hipStream_t st, double* base, const void* descs, int n, int* f
 }
 
 // ---- eigenvectors of a Gram matrix by one-sided Jacobi without accumulation --
-// One work-group per k x k Gram matrix G (k <= kGramEigMaxK) held in LDS.  The
-// one-sided Jacobi drives X = G W to mutually orthogonal columns; since G is
-// symmetric positive semi-definite, X = W diag(lambda) at convergence, so W is
-// recovered as the normalised columns of X and no rotation is accumulated.
-// Rotations are made down to tol/16, so the columns of X end orthonormal well
-// inside tol whatever the accuracy of the small eigenpairs; exactly zero
-// columns are completed from the unit vector with the largest residual
-// against the others (two projection passes).
+// One work-group per k x k Gram matrix G (k <= kGramEigMaxK) held in LDS.
 //
-// Each column pair of a round is owned by gsz lanes that keep its rows in
-// registers (kEigR per lane): one LDS read and one write per element per round.
-//
-// flags[flag_idx] = number of column pairs of G that fail the tol test in the
-// first sweep; when it is 0 the input was already diagonal to tol, the sweeps
-// stop and W = I exactly.
+// 1. Test: the number of pairs with |g_ij| > tol sqrt(g_ii g_jj) goes to
+//    flags[flag_idx].  When it is 0 the columns behind G are orthogonal to
+//    tol and W = I exactly.
+// 2. Preconditioning (precond != 0): pivoted Cholesky P^T G P = R^T R in
+//    place (diagonal pivoting, stops at a non-positive pivot), X = R^T.  This
+//    is the QR-with-pivoting preconditioner of Drmac and Veselic (SIAM J.
+//    Matrix Anal. Appl. 29, 2008) applied through G: on graded spectra the
+//    Jacobi below then needs a few sweeps instead of tens.  Without it X = G.
+// 3. One-sided Jacobi drives X W to mutually orthogonal columns.  Since
+//    X X^T = P^T G P (or G^2), the normalised columns of the result are the
+//    eigenvectors of P^T G P and no rotation is accumulated.  Rotations are
+//    made down to tol/16, so the columns end orthonormal well inside tol
+//    whatever the accuracy of the small eigenpairs; zero columns are
+//    completed from the unit vector with the largest residual (two
+//    projection passes).
+// 4. W is written with the rows un-permuted (W = P Q).
+// Each column pair of a Jacobi round is owned by gsz lanes that keep its rows
+// in registers (kEigR per lane): one LDS read and one write per element.
 constexpr int kEigR = 18;
+constexpr int kCholPer = (kGramEigMaxK * kGramEigMaxK + kSmallThreads - 1) / kSmallThreads;
 
 __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__ base,
                                                             const SmallJacobi* __restrict__ js,
-                                                            int* __restrict__ flags, double tol, int max_sweeps) {
+                                                            int* __restrict__ flags, double tol, int max_sweeps,
+                                                            int precond) {
   extern __shared__ double lds[];
   __shared__ int nrot[2];
-  __shared__ int nviol, best;
+  __shared__ int nviol, best, piv, rank;
+  __shared__ double pmax;
   const SmallJacobi J = js[blockIdx.x];
   const int k = J.k;
   const int ke = k + (k & 1);
@@ -374,6 +382,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
   double* nrm = lds + int64_t(k) * ke;     // ke
   double* vec = nrm + ke;                  // ke
   double* hh = vec + ke;                   // ke
+  int* perm = reinterpret_cast<int*>(hh + ke);  // ke
   double* Gg = base + J.y_off;
   double* Wg = base + J.w_off;
   const int npairs = ke / 2;
@@ -390,15 +399,103 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
     const int r = e % k, c = e / k;
     X[e] = c < k ? Gg[int64_t(c) * J.ldy + r] : 0.0;
   }
+  for (int e = tid; e < ke; e += blockDim.x) perm[e] = e;
   if (tid == 0) {
     nviol = 0;
     nrot[0] = nrot[1] = 0;
+    rank = k;
   }
   __syncthreads();
+  // ---- 1. orthogonality test on G -------------------------------------------
+  {
+    int my = 0;
+    for (int e = tid; e < k * k; e += blockDim.x) {
+      const int r = e % k, c = e / k;
+      if (r < c) {
+        const double gii = X[r * k + r], gjj = X[c * k + c], gij = X[e];
+        if (gii > 0.0 && gjj > 0.0 && fabs(gij) > tol * sqrt(gii) * sqrt(gjj)) ++my;
+      }
+    }
+    if (my) atomicAdd(&nviol, my);
+  }
+  __syncthreads();
+  const bool ident = nviol == 0 || k < 2;
+  if (!ident && precond) {
+    // ---- 2. pivoted Cholesky, right-looking, in place ------------------------
+    // After step j, column perm[j] of X holds L(:, j) at rows perm[i], i >= j.
+    for (int j = 0; j < k; ++j) {
+      if (tid < 64) {  // pivot: largest remaining diagonal, smallest position on ties
+        double bv = -1.0;
+        int bi = k;
+        for (int i = j + tid; i < k; i += 64) {
+          const int pi_ = perm[i];
+          const double v = X[pi_ * k + pi_];
+          if (v > bv) {
+            bv = v;
+            bi = i;
+          }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(bv, off);
+          const int oi = __shfl_xor(bi, off);
+          if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+          }
+        }
+        if (tid == 0) {
+          pmax = bv;
+          piv = bi;
+        }
+      }
+      __syncthreads();
+      if (!(pmax > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
+        if (tid == 0) rank = j;
+        break;
+      }
+      if (tid == 0) {
+        const int t = perm[j];
+        perm[j] = perm[piv];
+        perm[piv] = t;
+      }
+      __syncthreads();
+      const int pj = perm[j];
+      const double ljj = sqrt(pmax);
+      for (int i = j + 1 + tid; i < k; i += blockDim.x) X[pj * k + perm[i]] /= ljj;
+      __syncthreads();
+      if (tid == 0) X[pj * k + pj] = ljj;
+      const int m = k - j - 1;
+      for (int e = tid; e < m * m; e += blockDim.x) {
+        const int ii = perm[j + 1 + e % m], ll = perm[j + 1 + e / m];
+        X[ll * k + ii] -= X[pj * k + ii] * X[pj * k + ll];
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    // X = L (lower triangular, permuted coordinates): gather through registers
+    const int rk = rank;
+    double tmp[kCholPer];
+#pragma unroll
+    for (int q = 0; q < kCholPer; ++q) {
+      const int e = tid + q * kSmallThreads;
+      tmp[q] = 0.0;
+      if (e < k * k) {
+        const int i = e % k, jc = e / k;
+        if (i >= jc && jc < rk) tmp[q] = X[perm[jc] * k + perm[i]];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kCholPer; ++q) {
+      const int e = tid + q * kSmallThreads;
+      if (e < k * k) X[e] = tmp[q];
+    }
+    __syncthreads();
+  }
+  // ---- 3. one-sided Jacobi on the columns of X ------------------------------
   const double tol_rot = tol * (1.0 / 16.0);
   const int pi = tid / gsz, li = tid % gsz;
-  bool ident = k < 2;
-  for (int sweep = 0; sweep < max_sweeps && k >= 2; ++sweep) {
+  for (int sweep = 0; sweep < max_sweeps && !ident; ++sweep) {
     int* cnt = &nrot[sweep & 1];
     for (int r = 0; r < ke - 1; ++r) {
       if (pi < npairs) {
@@ -424,38 +521,30 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
           be += __shfl_xor(be, off, gsz);
           ga += __shfl_xor(ga, off, gsz);
         }
-        if (al > 0.0 && be > 0.0) {
-          const double lim = sqrt(al) * sqrt(be);
-          if (sweep == 0 && li == 0 && fabs(ga) > tol * lim) atomicAdd(&nviol, 1);
-          if (fabs(ga) > tol_rot * lim) {
-            const double zeta = (be - al) / (2.0 * ga);
-            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+        if (al > 0.0 && be > 0.0 && fabs(ga) > tol_rot * (sqrt(al) * sqrt(be))) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
 #pragma unroll
-            for (int j = 0; j < kEigR; ++j) {
-              const int l = li + j * gsz;
-              if (l < k) {
-                ya[l] = c * xa[j] - sn * xb[j];
-                yb[l] = sn * xa[j] + c * xb[j];
-              }
+          for (int j = 0; j < kEigR; ++j) {
+            const int l = li + j * gsz;
+            if (l < k) {
+              ya[l] = c * xa[j] - sn * xb[j];
+              yb[l] = sn * xa[j] + c * xb[j];
             }
-            if (li == 0) atomicAdd(cnt, 1);
           }
+          if (li == 0) atomicAdd(cnt, 1);
         }
       }
       __syncthreads();
     }
     const int n = *cnt;
-    const bool clean = sweep == 0 && nviol == 0;
     if (tid == 0) nrot[(sweep + 1) & 1] = 0;
     __syncthreads();
-    if (clean) {
-      ident = true;
-      break;
-    }
     if (n == 0) break;
   }
   if (!ident) {
+    // ---- 4. normalise, complete zero columns --------------------------------
     // column norms: the pairs of round 0 cover every column once
     if (pi < npairs) {
       const int a = circle_block(pi, 0, ke), b = circle_block(ke - 1 - pi, 0, ke);
@@ -523,23 +612,27 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
       __syncthreads();
     }
   }
+  // ---- 5. W = P Q ---------------------------------------------------------------
   for (int e = tid; e < k * k; e += blockDim.x) {
     const int r = e % k, c = e / k;
-    Wg[int64_t(c) * J.ldw + r] = ident ? (r == c ? 1.0 : 0.0) : X[e];
+    if (ident)
+      Wg[int64_t(c) * J.ldw + r] = (r == c) ? 1.0 : 0.0;
+    else
+      Wg[int64_t(c) * J.ldw + perm[r]] = X[e];
   }
   if (tid == 0 && J.flag_idx >= 0) flags[J.flag_idx] = nviol;
 }
 
 void gram_eig(hipStream_t st, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
-              size_t lds_bytes) {
+              int precond, size_t lds_bytes) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_gram_eig, dim3(n), dim3(kSmallThreads), lds_bytes, st, base,
-                     static_cast<const SmallJacobi*>(descs), flags, tol, max_sweeps);
+                     static_cast<const SmallJacobi*>(descs), flags, tol, max_sweeps, precond);
 }
 
 size_t gram_eig_lds_bytes(int k) {
   const int64_t ke = k + (k & 1);
-  return size_t(int64_t(k) * ke + 3 * ke) * sizeof(double);
+  return size_t(int64_t(k) * ke + 3 * ke) * sizeof(double) + size_t(ke) * sizeof(int);
 }
 
 size_t jacobi_small_lds_limit() {

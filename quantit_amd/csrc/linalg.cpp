@@ -277,7 +277,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   for (auto& s : B.secs) max_ell = std::max(max_ell, s.ell);
   const double tol = 8.0 * std::sqrt(double(std::max(max_ell, 16))) * DBL_EPSILON;
   const int inner = env_int("QTT_SVD_INNER_SWEEPS", 1);
-  const int max_sweeps = 30;
+  const int max_sweeps = env_int("QTT_SVD_MAX_SWEEPS", 30);
   const size_t lds_limit = kern::jacobi_small_lds_limit();
   const size_t lds_margin = 256;
   auto lds_bytes = [](int64_t ell, int64_t k) { return size_t(ell * (k + (k & 1))) * sizeof(double); };
@@ -402,7 +402,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         for (size_t j = 0; j < act.size(); ++j) h_flags[act[j]] = fr[j];
       } else {
         KernelTimer kt(*ctx, "svd_eig");
-        kern::gram_eig(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), d_flags, tol, max_sweeps, lds_need);
+        kern::gram_eig(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), d_flags, tol, max_sweeps,
+                       env_int("QTT_SVD_PRECOND", 1), lds_need);
       }
       if (it >= 1) {  // G diagonal to tol: the columns of Y are orthogonal, the sector is done
         if (!gram_multi) {
