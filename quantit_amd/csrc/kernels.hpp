@@ -132,7 +132,8 @@ void fill_i32(hipStream_t s, int32_t* p, int n, int32_t v);
 struct SmallJacobi {
   int64_t y_off, w_off;  // Y (ell x k, column-major, ld ldy), W (ldw x ldw, column-major)
   int32_t ell, k, ldy, ldw;
-  int32_t flag_idx, pad;  // flags[flag_idx] = rotations of the first sweep (-1: none)
+  int32_t flag_idx;       // flags[flag_idx]: result count of the kernel (-1: none)
+  int32_t aux_idx;        // flags[aux_idx]: second count (gram_eig; -1: none)
 };
 void jacobi_small(hipStream_t s, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
                   size_t lds_bytes);
@@ -144,7 +145,10 @@ size_t jacobi_small_lds_limit();
 constexpr int kGramEigMaxK = 140;
 void gram_eig(hipStream_t s, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
               int precond, size_t lds_bytes);
-size_t gram_eig_lds_bytes(int k);  // largest dynamic LDS per work-group (bytes)
+size_t gram_eig_lds_bytes(int k);
+// First-order Gram-Schmidt factor T (k x k at w_off) from the Gram matrix at
+// y_off (SmallJacobi with ell = k, k <= kGramEigMaxK).
+void gs_step(hipStream_t s, double* base, const void* descs, int n);  // largest dynamic LDS per work-group (bytes)
 
 // Global truncation (sources/btensor_linalg.cpp:657-755,
 // sources/LinearAlgebra.cpp:57-75): singular values of nsec sectors stored

@@ -18,6 +18,7 @@
 // the accuracy class of LAPACK gesdd used by the reference
 // (sources/btensor_linalg.cpp:483).
 #include <cfloat>
+#include <climits>
 
 #include "../kernels.hpp"
 
@@ -345,9 +346,13 @@ void jacobi_small(hipStream_t st, double* base, const void* descs, int n, int* f
 // ---- eigenvectors of a Gram matrix by one-sided Jacobi without accumulation --
 // One work-group per k x k Gram matrix G (k <= kGramEigMaxK) held in LDS.
 //
-// 1. Test: the number of pairs with |g_ij| > tol sqrt(g_ii g_jj) goes to
-//    flags[flag_idx].  When it is 0 the columns behind G are orthogonal to
-//    tol and W = I exactly.
+// 1. Test: a pair (i, j) of the columns behind G is converged when
+//    |g_ij| <= tol sqrt(g_ii g_jj) (relative orthogonality) or
+//    |g_ij| <= tol sqrt(max(g_ii, g_jj)) sqrt(max_l g_ll) (its coupling moves
+//    the larger column's share of the smaller one by at most tol times the
+//    largest singular value: absolute accuracy, the class of LAPACK gesdd).
+//    flags[flag_idx] = pairs failing both; flags[J.aux_idx] (when >= 0) = pairs
+//    converged only in the absolute sense.  No failing pair: W = I exactly.
 // 2. Preconditioning (precond != 0): pivoted Cholesky P^T G P = R^T R in
 //    place (diagonal pivoting, stops at a non-positive pivot), X = R^T.  This
 //    is the QR-with-pivoting preconditioner of Drmac and Veselic (SIAM J.
@@ -364,7 +369,62 @@ void jacobi_small(hipStream_t st, double* base, const void* descs, int n, int* f
 // Each column pair of a Jacobi round is owned by gsz lanes that keep its rows
 // in registers (kEigR per lane): one LDS read and one write per element.
 constexpr int kEigR = 18;
-constexpr int kCholPer = (kGramEigMaxK * kGramEigMaxK + kSmallThreads - 1) / kSmallThreads;
+
+// Sum over aligned groups of g lanes (g a power of two <= 16) with DPP moves:
+// xor 1 and xor 2 by quad permutes, then the 8- and 16-lane mirrors (after
+// the previous steps every lane of a half holds the same partial sum, so the
+// mirror partner holds the other half's).
+__device__ __forceinline__ double dpp_bits(double v, int ctrl_id) {
+  const long long b = __double_as_longlong(v);
+  int lo = int(b & 0xffffffffLL), hi = int(b >> 32);
+  switch (ctrl_id) {
+    case 0:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+      break;
+    case 1:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
+      break;
+    case 2:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false);
+      break;
+    default:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0x140, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0x140, 0xF, 0xF, false);
+      break;
+  }
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+__device__ __forceinline__ void group_sum3(double& a, double& b, double& c, int g) {
+  if (g > 16) {
+    for (int off = g >> 1; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, g);
+      b += __shfl_xor(b, off, g);
+      c += __shfl_xor(c, off, g);
+    }
+    return;
+  }
+  for (int st = 0; (2 << st) <= g; ++st) {
+    const double ta = dpp_bits(a, st), tb = dpp_bits(b, st), tc = dpp_bits(c, st);
+    a += ta;
+    b += tb;
+    c += tc;
+  }
+}
+
+// Lanes per column pair: enough for about 16 rows per lane, at most what the
+// block holds for all pairs.
+__host__ __device__ inline int gram_eig_gsz(int k, int threads) {
+  const int npairs = (k + (k & 1)) / 2;
+  int g = 1;
+  while (g < 16 && g * 16 < k) g <<= 1;
+  while (g > 1 && g * npairs > threads) g >>= 1;
+  return g;
+}
+
 
 __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__ base,
                                                             const SmallJacobi* __restrict__ js,
@@ -372,7 +432,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
                                                             int precond) {
   extern __shared__ double lds[];
   __shared__ int nrot[2];
-  __shared__ int nviol, best, piv, rank;
+  __shared__ int nviol, nabs, best, piv, rank;
   __shared__ double pmax;
   const SmallJacobi J = js[blockIdx.x];
   const int k = J.k;
@@ -386,8 +446,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
   double* Gg = base + J.y_off;
   double* Wg = base + J.w_off;
   const int npairs = ke / 2;
-  int gsz = 64;
-  while (gsz > 1 && gsz * npairs > int(blockDim.x)) gsz >>= 1;
+  const int gsz = gram_eig_gsz(k, int(blockDim.x));
   if (k <= 0) return;
   if (k > kGramEigMaxK || (k + gsz - 1) / gsz > kEigR) {  // never launched so by the host
     for (int64_t e = tid; e < int64_t(k) * k; e += blockDim.x)
@@ -402,50 +461,70 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
   for (int e = tid; e < ke; e += blockDim.x) perm[e] = e;
   if (tid == 0) {
     nviol = 0;
+    nabs = 0;
     nrot[0] = nrot[1] = 0;
     rank = k;
   }
   __syncthreads();
   // ---- 1. orthogonality test on G -------------------------------------------
   {
-    int my = 0;
+    double dm = 0.0;
+    for (int i = tid; i < k; i += blockDim.x) dm = fmax(dm, X[i * k + i]);
+    if (tid == 0) nrm[0] = 0.0;  // scratch for the maximum (nrm is free until the Jacobi ends)
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) dm = fmax(dm, __shfl_xor(dm, off));
+    if ((tid & 63) == 0) atomicMax(reinterpret_cast<unsigned long long*>(nrm),
+                                   static_cast<unsigned long long>(__double_as_longlong(dm)));
+    __syncthreads();
+    const double sgmax = sqrt(nrm[0]);
+    int my = 0, mya = 0;
     for (int e = tid; e < k * k; e += blockDim.x) {
       const int r = e % k, c = e / k;
       if (r < c) {
-        const double gii = X[r * k + r], gjj = X[c * k + c], gij = X[e];
-        if (gii > 0.0 && gjj > 0.0 && fabs(gij) > tol * sqrt(gii) * sqrt(gjj)) ++my;
+        const double gii = X[r * k + r], gjj = X[c * k + c], g = fabs(X[e]);
+        if (gii > 0.0 && gjj > 0.0 && g > tol * sqrt(gii) * sqrt(gjj)) {
+          if (g > tol * sqrt(fmax(gii, gjj)) * sgmax)
+            ++my;
+          else
+            ++mya;
+        }
       }
     }
     if (my) atomicAdd(&nviol, my);
+    if (mya) atomicAdd(&nabs, mya);
   }
   __syncthreads();
   const bool ident = nviol == 0 || k < 2;
   if (!ident && precond) {
     // ---- 2. pivoted Cholesky, right-looking, in place ------------------------
-    // After step j, column perm[j] of X holds L(:, j) at rows perm[i], i >= j.
+    // Rows and columns are swapped physically, so after step j column j holds
+    // L(j:k, j) below the diagonal and the trailing block is contiguous; perm
+    // maps positions to the original indices; vec holds the trailing diagonal.
+    for (int i = tid; i < k; i += blockDim.x) vec[i] = X[i * k + i];
+    __syncthreads();
     for (int j = 0; j < k; ++j) {
-      if (tid < 64) {  // pivot: largest remaining diagonal, smallest position on ties
-        double bv = -1.0;
-        int bi = k;
+      if (tid < 64) {
+        // pivot: largest remaining diagonal, smallest position on ties.  The
+        // key keeps the top 54 bits of the value and the inverted position in
+        // the low 10 bits (k <= 1023); non-positive values never win over
+        // positive ones under a signed compare.
+        long long key = LLONG_MIN;
         for (int i = j + tid; i < k; i += 64) {
-          const int pi_ = perm[i];
-          const double v = X[pi_ * k + pi_];
-          if (v > bv) {
-            bv = v;
-            bi = i;
-          }
+          const long long kk = (__double_as_longlong(vec[i]) & ~0x3FFLL) | (1023 - i);
+          key = kk > key ? kk : key;
         }
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(bv, off);
-          const int oi = __shfl_xor(bi, off);
-          if (ov > bv || (ov == bv && oi < bi)) {
-            bv = ov;
-            bi = oi;
-          }
+        for (int st = 0; st < 4; ++st) {
+          const long long o = __double_as_longlong(dpp_bits(__longlong_as_double(key), st));
+          key = o > key ? o : key;
+        }
+        for (int off = 16; off < 64; off <<= 1) {
+          const long long o = __shfl_xor(key, off);
+          key = o > key ? o : key;
         }
         if (tid == 0) {
-          pmax = bv;
-          piv = bi;
+          const int pi_ = key == LLONG_MIN ? j : 1023 - int(key & 0x3FF);
+          piv = pi_;
+          pmax = vec[pi_];
         }
       }
       __syncthreads();
@@ -453,42 +532,52 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
         if (tid == 0) rank = j;
         break;
       }
-      if (tid == 0) {
-        const int t = perm[j];
-        perm[j] = perm[piv];
-        perm[piv] = t;
+      const int pv = piv;
+      if (pv != j) {  // symmetric swap of position j and pv (uniform branch)
+        for (int r = tid; r < k; r += blockDim.x) {
+          const double t = X[j * k + r];
+          X[j * k + r] = X[pv * k + r];
+          X[pv * k + r] = t;
+        }
+        __syncthreads();
+        for (int c = tid; c < k; c += blockDim.x) {
+          const double t = X[c * k + j];
+          X[c * k + j] = X[c * k + pv];
+          X[c * k + pv] = t;
+        }
+        if (tid == 0) {
+          const double t = vec[j];
+          vec[j] = vec[pv];
+          vec[pv] = t;
+          const int q = perm[j];
+          perm[j] = perm[pv];
+          perm[pv] = q;
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      const int pj = perm[j];
-      const double ljj = sqrt(pmax);
-      for (int i = j + 1 + tid; i < k; i += blockDim.x) X[pj * k + perm[i]] /= ljj;
-      __syncthreads();
-      if (tid == 0) X[pj * k + pj] = ljj;
+      const double inv = 1.0 / pmax;
       const int m = k - j - 1;
       for (int e = tid; e < m * m; e += blockDim.x) {
-        const int ii = perm[j + 1 + e % m], ll = perm[j + 1 + e / m];
-        X[ll * k + ii] -= X[pj * k + ii] * X[pj * k + ll];
+        const int ii = j + 1 + e % m, ll = j + 1 + e / m;
+        X[ll * k + ii] -= X[j * k + ii] * X[j * k + ll] * inv;
       }
+      __syncthreads();
+      const double ljj = sqrt(pmax);
+      for (int i = j + 1 + tid; i < k; i += blockDim.x) {
+        const double l = X[j * k + i] / ljj;
+        X[j * k + i] = l;
+        vec[i] -= l * l;
+      }
+      if (tid == 0) X[j * k + j] = ljj;
       __syncthreads();
     }
     __syncthreads();
-    // X = L (lower triangular, permuted coordinates): gather through registers
-    const int rk = rank;
-    double tmp[kCholPer];
-#pragma unroll
-    for (int q = 0; q < kCholPer; ++q) {
-      const int e = tid + q * kSmallThreads;
-      tmp[q] = 0.0;
-      if (e < k * k) {
+    {  // X = L: keep the lower triangle of the first rank columns
+      const int rk = rank;
+      for (int e = tid; e < k * k; e += blockDim.x) {
         const int i = e % k, jc = e / k;
-        if (i >= jc && jc < rk) tmp[q] = X[perm[jc] * k + perm[i]];
+        if (i < jc || jc >= rk) X[e] = 0.0;
       }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kCholPer; ++q) {
-      const int e = tid + q * kSmallThreads;
-      if (e < k * k) X[e] = tmp[q];
     }
     __syncthreads();
   }
@@ -516,14 +605,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
           be += xb[j] * xb[j];
           ga += xa[j] * xb[j];
         }
-        for (int off = gsz >> 1; off > 0; off >>= 1) {
-          al += __shfl_xor(al, off, gsz);
-          be += __shfl_xor(be, off, gsz);
-          ga += __shfl_xor(ga, off, gsz);
-        }
-        if (al > 0.0 && be > 0.0 && fabs(ga) > tol_rot * (sqrt(al) * sqrt(be))) {
-          const double zeta = (be - al) / (2.0 * ga);
-          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        group_sum3(al, be, ga, gsz);
+        if (al > 0.0 && be > 0.0 && ga * ga > (tol_rot * al) * (tol_rot * be)) {
+          // t = tan(theta) of the rotation annihilating ga, |theta| <= pi/4:
+          // t = sign(d) 2 ga / (|d| + sqrt(d^2 + 4 ga^2)), d = be - al
+          const double d = be - al;
+          const double t = (d >= 0.0 ? 2.0 * ga : -2.0 * ga) / (fabs(d) + sqrt(d * d + 4.0 * ga * ga));
           const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
 #pragma unroll
           for (int j = 0; j < kEigR; ++j) {
@@ -549,14 +636,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
     if (pi < npairs) {
       const int a = circle_block(pi, 0, ke), b = circle_block(ke - 1 - pi, 0, ke);
       double al = 0.0, be = 0.0;
+      double zero = 0.0;
       for (int l = li; l < k; l += gsz) {
         al += X[a * k + l] * X[a * k + l];
         be += X[b * k + l] * X[b * k + l];
       }
-      for (int off = gsz >> 1; off > 0; off >>= 1) {
-        al += __shfl_xor(al, off, gsz);
-        be += __shfl_xor(be, off, gsz);
-      }
+      group_sum3(al, be, zero, gsz);
       if (li == 0) {
         nrm[a] = sqrt(al);
         nrm[b] = sqrt(be);
@@ -621,6 +706,33 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
       Wg[int64_t(c) * J.ldw + perm[r]] = X[e];
   }
   if (tid == 0 && J.flag_idx >= 0) flags[J.flag_idx] = nviol;
+  if (tid == 0 && J.aux_idx >= 0) flags[J.aux_idx] = nabs;
+}
+
+// First-order Gram-Schmidt step on columns whose Gram matrix is G (k x k,
+// leading dimension ldy): T = I - M with M_ij = g_ij / g_ii when column i is
+// longer than column j (ties: lower index), so Z T removes from every column
+// its components along the longer ones.  T is written k x k at w_off
+// (leading dimension ldw).  One work-group per matrix.
+__global__ __launch_bounds__(256) void k_gs_step(double* __restrict__ base, const SmallJacobi* __restrict__ js) {
+  __shared__ double d[kGramEigMaxK];
+  const SmallJacobi J = js[blockIdx.x];
+  const int k = J.k;
+  const double* G = base + J.y_off;
+  double* T = base + J.w_off;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) d[i] = G[int64_t(i) * J.ldy + i];
+  __syncthreads();
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    const int i = e % k, j = e / k;
+    double v = i == j ? 1.0 : 0.0;
+    if (i != j && d[i] > 0.0 && (d[i] > d[j] || (d[i] == d[j] && i < j))) v = -G[int64_t(j) * J.ldy + i] / d[i];
+    T[int64_t(j) * J.ldw + i] = v;
+  }
+}
+
+void gs_step(hipStream_t st, double* base, const void* descs, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gs_step, dim3(n), dim3(256), 0, st, base, static_cast<const SmallJacobi*>(descs));
 }
 
 void gram_eig(hipStream_t st, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,

@@ -355,11 +355,11 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       alt_y[q] = gb + y2o[q];
       alt_w[q] = gb + w2o[q];
     }
-    BufPtr fbuf = ctx->alloc(size_t(nw) * sizeof(int32_t));
+    BufPtr fbuf = ctx->alloc(size_t(2 * nw) * sizeof(int32_t));
     keep.push_back(fbuf);
     int32_t* d_flags = reinterpret_cast<int32_t*>(fbuf->cptr());
-    std::vector<int32_t> h_flags(nw);
-    std::vector<int> act(nw);
+    std::vector<int32_t> h_flags(size_t(2 * nw), 0);
+    std::vector<int> act(nw), corr;
     std::iota(act.begin(), act.end(), 0);
     for (int it = 0; it < 6 && !act.empty(); ++it) {
       // G = Y^T Y over the k real columns; its eigenvectors W_g in one work-group
@@ -378,7 +378,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         gp.K = s.ell;
         outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
         prs.push_back(gp);
-        jd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, q, 0});
+        jd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, q, nw + q});
       }
       launch_gemm(ctx, ab, outs, prs, "svd_gram");
       if (gram_multi) {
@@ -401,22 +401,25 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         jacobi_sweeps(ctx, ab, gsec, all, tol, inner, max_sweeps, &fr);
         for (size_t j = 0; j < act.size(); ++j) h_flags[act[j]] = fr[j];
       } else {
-        KernelTimer kt(*ctx, "svd_eig");
+        static const char* const kEigNames[] = {"svd_eig", "svd_eig_it1", "svd_eig_it2", "svd_eig_it3+"};
+        KernelTimer kt(*ctx, kEigNames[std::min(it, 3)]);
         kern::gram_eig(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), d_flags, tol, max_sweeps,
                        env_int("QTT_SVD_PRECOND", 1), lds_need);
       }
       if (it >= 1) {  // G diagonal to tol: the columns of Y are orthogonal, the sector is done
         if (!gram_multi) {
-          QTT_HIP(hipMemcpyAsync(h_flags.data(), d_flags, size_t(nw) * sizeof(int32_t), hipMemcpyDeviceToHost,
+          QTT_HIP(hipMemcpyAsync(h_flags.data(), d_flags, size_t(2 * nw) * sizeof(int32_t), hipMemcpyDeviceToHost,
                                  ctx->stream()));
           ctx->sync();
         }
         std::vector<int> next;
         for (int q : act) {
-          if (h_flags[q] == 0)
+          if (h_flags[q] == 0) {
             done[wide[q]] = 1;
-          else
+            if (h_flags[nw + q] > 0) corr.push_back(q);  // converged in the absolute sense only
+          } else {
             next.push_back(q);
+          }
         }
         act.swap(next);
         if (act.empty()) break;
@@ -452,6 +455,56 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       B.sweeps += 1;
     }
     for (int q : act) big_only[wide[q]] = 1;  // keeps its accumulated W: multi-WG path only
+    // Sectors converged with residual couplings below the absolute floor:
+    // two first-order Gram-Schmidt passes Y <- Y T (longer columns first)
+    // make the short columns orthogonal; W is unchanged, and the dropped
+    // triangular part is below tol times the largest singular value.
+    for (int pass = 0; pass < 2 && !corr.empty(); ++pass) {
+      std::vector<kern::GemmOut> outs;
+      std::vector<kern::GemmPair> prs;
+      if (pass > 0) {  // the first pass reuses the last Gram matrix of the loop
+        for (int q : corr) {
+          const auto& s = B.secs[wide[q]];
+          kern::GemmPair gp{};
+          gp.a_off = s.y_off;
+          gp.a_rs = s.ell;
+          gp.a_cs = 1;
+          gp.b_off = s.y_off;
+          gp.b_rs = 1;
+          gp.b_cs = s.ell;
+          gp.K = s.ell;
+          outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
+          prs.push_back(gp);
+        }
+        launch_gemm(ctx, ab, outs, prs, "svd_gram");
+        outs.clear();
+        prs.clear();
+      }
+      std::vector<kern::SmallJacobi> gd;
+      for (int q : corr) {
+        const auto& s = B.secs[wide[q]];
+        gd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, -1, -1});
+      }
+      {
+        KernelTimer kt(*ctx, "svd_gs");
+        kern::gs_step(ctx->stream(), ab, ctx->upload_vec(gd), int(gd.size()));
+      }
+      for (int q : corr) {
+        const auto& s = B.secs[wide[q]];
+        kern::GemmPair p1{};
+        p1.a_off = gb + gwo[q];
+        p1.a_rs = s.k;
+        p1.a_cs = 1;
+        p1.b_off = s.y_off;
+        p1.b_rs = s.ell;
+        p1.b_cs = 1;
+        p1.K = s.k;
+        outs.push_back({alt_y[q], s.k, s.ell, int32_t(prs.size()), int32_t(prs.size() + 1), s.ell, 0});
+        prs.push_back(p1);
+      }
+      launch_gemm(ctx, ab, outs, prs, "svd_rotate");
+      for (int q : corr) std::swap(B.secs[wide[q]].y_off, alt_y[q]);
+    }
   }
   // ---- Jacobi on everything not yet orthogonal --------------------------------
   std::vector<int> rest_small, rest_big;
