@@ -147,8 +147,28 @@ void gram_eig(hipStream_t s, double* base, const void* descs, int n, int* flags,
               int precond, size_t lds_bytes);
 size_t gram_eig_lds_bytes(int k);
 // First-order Gram-Schmidt factor T (k x k at w_off) from the Gram matrix at
-// y_off (SmallJacobi with ell = k, k <= kGramEigMaxK).
-void gs_step(hipStream_t s, double* base, const void* descs, int n);  // largest dynamic LDS per work-group (bytes)
+// y_off (SmallJacobi with ell = k).
+void gs_step(hipStream_t s, double* base, const void* descs, int n);
+
+// ---- Gram eigenvectors for wide sectors (G in device memory) ---------------
+// Orthogonality test of G (SmallJacobi: y_off = G, k, ldy; flag_idx/aux_idx
+// as for gram_eig).
+void gram_check(hipStream_t s, const double* base, const void* descs, int n, int* flags, double tol);
+struct CholPanel {
+  int64_t g_off;      // G (k x k, ld k), updated between panels
+  int64_t l_off;      // L (k x k, ld k, zeroed), rows in the original order
+  int64_t lp_off;     // panel copy (k x nb)
+  int64_t state_off;  // ints: [0] rank (k while running), [1 + i] index i pivoted
+  int32_t k, pad;
+};
+// One panel of nb columns starting at j0; LDS kmax * nb doubles.
+void pchol_panel(hipStream_t s, double* base, const void* descs, int n, int j0, int nb, int kmax);
+size_t pchol_lds_limit();
+struct GramVec {
+  int64_t x_off, w_off, scratch_off;  // scratch: 3k doubles
+  int32_t k, ldx, ldw, pad;
+};
+void gram_vectors(hipStream_t s, double* base, const void* descs, int n);  // largest dynamic LDS per work-group (bytes)
 
 // Global truncation (sources/btensor_linalg.cpp:657-755,
 // sources/LinearAlgebra.cpp:57-75): singular values of nsec sectors stored

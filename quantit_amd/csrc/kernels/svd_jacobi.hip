@@ -715,17 +715,17 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
 // its components along the longer ones.  T is written k x k at w_off
 // (leading dimension ldw).  One work-group per matrix.
 __global__ __launch_bounds__(256) void k_gs_step(double* __restrict__ base, const SmallJacobi* __restrict__ js) {
-  __shared__ double d[kGramEigMaxK];
   const SmallJacobi J = js[blockIdx.x];
   const int k = J.k;
   const double* G = base + J.y_off;
   double* T = base + J.w_off;
-  for (int i = threadIdx.x; i < k; i += blockDim.x) d[i] = G[int64_t(i) * J.ldy + i];
-  __syncthreads();
-  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
-    const int i = e % k, j = e / k;
+  for (int64_t e = threadIdx.x; e < int64_t(k) * k; e += blockDim.x) {
+    const int i = int(e % k), j = int(e / k);
     double v = i == j ? 1.0 : 0.0;
-    if (i != j && d[i] > 0.0 && (d[i] > d[j] || (d[i] == d[j] && i < j))) v = -G[int64_t(j) * J.ldy + i] / d[i];
+    if (i != j) {
+      const double di = G[int64_t(i) * J.ldy + i], dj = G[int64_t(j) * J.ldy + j];
+      if (di > 0.0 && (di > dj || (di == dj && i < j))) v = -G[int64_t(j) * J.ldy + i] / di;
+    }
     T[int64_t(j) * J.ldw + i] = v;
   }
 }

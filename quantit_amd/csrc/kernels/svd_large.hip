@@ -1,0 +1,289 @@
+// Gram-matrix eigenvectors for sectors wider than one work-group's LDS holds
+// (k > kGramEigMaxK).  G = Y^T Y (k x k, column-major, leading dimension k)
+// lives in device memory; the steps mirror the LDS solver of svd_jacobi.hip:
+//
+//   k_gram_check    orthogonality test of the columns behind G (relative, or
+//                   absolute with respect to the largest singular value),
+//   k_pchol_panel   one panel of nb columns of the pivoted Cholesky
+//                   P^T G P = R^T R (left-looking inside the panel; the
+//                   trailing matrix is updated by a GEMM G -= Lp Lp^T after
+//                   each panel).  L = R^T is kept with its rows in the
+//                   original order, so L L^T = G and the one-sided Jacobi on
+//                   the columns of L (block rounds of k_svd_round) yields G's
+//                   eigenvectors as the normalised columns,
+//   k_gram_vectors  normalises the columns of the rotated L into W and
+//                   completes zero columns from the unit vector with the
+//                   largest residual (two projection passes).
+#include <cfloat>
+#include <climits>
+
+#include "../kernels.hpp"
+
+namespace qtt {
+namespace kern {
+
+namespace {
+
+__device__ double block_max(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] = fmax(sh[tid], sh[tid + w]);
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] += sh[tid + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace
+
+constexpr int kLargeThreads = 1024;
+
+// flags[flag_idx] = pairs failing both tests, flags[aux_idx] = pairs passing
+// only the absolute one (same tests as k_gram_eig).
+__global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __restrict__ base,
+                                                              const SmallJacobi* __restrict__ js,
+                                                              int* __restrict__ flags, double tol) {
+  __shared__ double sh[kLargeThreads];
+  __shared__ int nv, na;
+  const SmallJacobi J = js[blockIdx.x];
+  const int k = J.k;
+  const double* G = base + J.y_off;
+  const int tid = threadIdx.x;
+  if (tid == 0) nv = na = 0;
+  double dm = 0.0;
+  for (int i = tid; i < k; i += blockDim.x) dm = fmax(dm, G[int64_t(i) * J.ldy + i]);
+  const double sgmax = sqrt(block_max(dm, sh));
+  int my = 0, mya = 0;
+  for (int64_t e = tid; e < int64_t(k) * k; e += blockDim.x) {
+    const int r = int(e % k), c = int(e / k);
+    if (r < c) {
+      const double gii = G[int64_t(r) * J.ldy + r], gjj = G[int64_t(c) * J.ldy + c];
+      const double g = fabs(G[int64_t(c) * J.ldy + r]);
+      if (gii > 0.0 && gjj > 0.0 && g > tol * sqrt(gii) * sqrt(gjj)) {
+        if (g > tol * sqrt(fmax(gii, gjj)) * sgmax)
+          ++my;
+        else
+          ++mya;
+      }
+    }
+  }
+  if (my) atomicAdd(&nv, my);
+  if (mya) atomicAdd(&na, mya);
+  __syncthreads();
+  if (tid == 0) {
+    if (J.flag_idx >= 0) flags[J.flag_idx] = nv;
+    if (J.aux_idx >= 0) flags[J.aux_idx] = na;
+  }
+}
+
+void gram_check(hipStream_t st, const double* base, const void* descs, int n, int* flags, double tol) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gram_check, dim3(n), dim3(kLargeThreads), 0, st, base,
+                     static_cast<const SmallJacobi*>(descs), flags, tol);
+}
+
+// One panel [j0, j0 + nb) of the pivoted Cholesky of G (already updated by
+// the previous panels).  state[0] = rank found so far (k while running),
+// state[1 + i] = 1 once original index i has been pivoted.  L is k x ldl
+// (column-major, leading dimension k), rows in the original order.  The
+// panel's columns are also written to Lp (k x nb, column-major) for the
+// trailing update.  LDS: nb columns of the panel, k doubles.
+__global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restrict__ base,
+                                                               const CholPanel* __restrict__ ps, int j0, int nb) {
+  extern __shared__ double lds[];
+  __shared__ double sh[kLargeThreads];
+  __shared__ int sh_i[kLargeThreads];
+  const CholPanel P = ps[blockIdx.x];
+  const int k = P.k;
+  const int tid = threadIdx.x;
+  const double* G = base + P.g_off;
+  double* L = base + P.l_off;
+  double* Lp = base + P.lp_off;
+  int* state = reinterpret_cast<int*>(base + P.state_off);
+  double* col = lds;  // nb columns of length k
+  if (state[0] < k || j0 >= k) {  // already rank-deficient or finished: zero panel
+    for (int64_t e = tid; e < int64_t(k) * nb; e += blockDim.x) Lp[e] = 0.0;
+    return;
+  }
+  for (int jj = 0; jj < nb; ++jj) {
+    const int j = j0 + jj;
+    if (j >= k) {
+      for (int i = tid; i < k; i += blockDim.x) col[jj * k + i] = 0.0;
+      continue;
+    }
+    // pivot: largest current diagonal among unused indices (G already holds
+    // the previous panels' updates; this panel's columns are subtracted here)
+    double bv = -1.0;
+    int bi = INT_MAX;
+    for (int i = tid; i < k; i += blockDim.x) {
+      if (state[1 + i]) continue;
+      double v = G[int64_t(i) * k + i];
+      for (int t = 0; t < jj; ++t) v -= col[t * k + i] * col[t * k + i];
+      if (v > bv || (v == bv && i < bi)) {
+        bv = v;
+        bi = i;
+      }
+    }
+    sh[tid] = bv;
+    sh_i[tid] = bi;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (tid < w) {
+        const double ov = sh[tid + w];
+        const int oi = sh_i[tid + w];
+        if (ov > sh[tid] || (ov == sh[tid] && oi < sh_i[tid])) {
+          sh[tid] = ov;
+          sh_i[tid] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    const double piv_v = sh[0];
+    const int p = sh_i[0];
+    __syncthreads();
+    if (!(piv_v > 0.0) || p == INT_MAX) {  // uniform: the rest is zero or negative noise
+      for (int t = jj; t < nb; ++t)
+        for (int i = tid; i < k; i += blockDim.x) col[t * k + i] = 0.0;
+      if (tid == 0) state[0] = j;
+      __syncthreads();
+      break;
+    }
+    const double ljj = sqrt(piv_v);
+    for (int i = tid; i < k; i += blockDim.x) {
+      double v = 0.0;
+      if (i == p) {
+        v = ljj;
+      } else if (!state[1 + i]) {
+        v = G[int64_t(p) * k + i];
+        for (int t = 0; t < jj; ++t) v -= col[t * k + i] * col[t * k + p];
+        v /= ljj;
+      }
+      col[jj * k + i] = v;
+    }
+    __syncthreads();
+    if (tid == 0) state[1 + p] = 1;
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int64_t e = tid; e < int64_t(k) * nb; e += blockDim.x) {
+    const int t = int(e / k), i = int(e % k);
+    Lp[e] = col[e];
+    if (j0 + t < k) L[int64_t(j0 + t) * k + i] = col[e];
+  }
+}
+
+void pchol_panel(hipStream_t st, double* base, const void* descs, int n, int j0, int nb, int kmax) {
+  if (n <= 0) return;
+  const size_t lds = size_t(kmax) * nb * sizeof(double);
+  hipLaunchKernelGGL(k_pchol_panel, dim3(n), dim3(kLargeThreads), lds, st, base,
+                     static_cast<const CholPanel*>(descs), j0, nb);
+}
+
+size_t pchol_lds_limit() {
+  static size_t limit = 0;
+  if (!limit) {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    limit = size_t(v) - 12 * 1024;  // static arrays of the kernel
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pchol_panel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit));
+  }
+  return limit;
+}
+
+// W = columns of X (k x k at x_off, leading dimension ldx) normalised; zero
+// columns completed.  W is written at w_off with leading dimension ldw;
+// scratch_off: 3k doubles.
+__global__ __launch_bounds__(kLargeThreads) void k_gram_vectors(double* __restrict__ base,
+                                                                const GramVec* __restrict__ js) {
+  __shared__ double sh[kLargeThreads];
+  __shared__ int best;
+  const GramVec J = js[blockIdx.x];
+  const int k = J.k;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const double* X = base + J.x_off;
+  double* W = base + J.w_off;
+  double* nrm = base + J.scratch_off;  // k
+  double* vec = nrm + k;               // k
+  double* hh = vec + k;                // k
+  for (int c = wave; c < k; c += nwave) {  // one wave per column
+    double a = 0.0;
+    for (int r = lane; r < k; r += 64) a += X[int64_t(c) * J.ldx + r] * X[int64_t(c) * J.ldx + r];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    if (lane == 0) nrm[c] = sqrt(a);
+  }
+  __syncthreads();
+  for (int64_t e = tid; e < int64_t(k) * k; e += blockDim.x) {
+    const int c = int(e / k), r = int(e % k);
+    const double v = nrm[c];
+    W[int64_t(c) * J.ldw + r] = v > 1e-300 ? X[int64_t(c) * J.ldx + r] / v : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < k; ++c) {
+    if (nrm[c] > 1e-300) continue;  // uniform
+    for (int r = tid; r < k; r += blockDim.x) {
+      double q = 1.0;
+      for (int j = 0; j < k; ++j) q -= W[int64_t(j) * J.ldw + r] * W[int64_t(j) * J.ldw + r];
+      vec[r] = q;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int bi = 0;
+      for (int r = 1; r < k; ++r)
+        if (vec[r] > vec[bi]) bi = r;
+      best = bi;
+    }
+    __syncthreads();
+    const int e0 = best;
+    for (int r = tid; r < k; r += blockDim.x) {
+      double q = r == e0 ? 1.0 : 0.0;
+      for (int j = 0; j < k; ++j) q -= W[int64_t(j) * J.ldw + r] * W[int64_t(j) * J.ldw + e0];
+      vec[r] = q;
+    }
+    __syncthreads();
+    for (int j = tid; j < k; j += blockDim.x) {
+      double q = 0.0;
+      for (int r = 0; r < k; ++r) q += W[int64_t(j) * J.ldw + r] * vec[r];
+      hh[j] = q;
+    }
+    __syncthreads();
+    for (int r = tid; r < k; r += blockDim.x) {
+      double q = vec[r];
+      for (int j = 0; j < k; ++j) q -= W[int64_t(j) * J.ldw + r] * hh[j];
+      vec[r] = q;
+    }
+    __syncthreads();
+    double a = 0.0;
+    for (int r = tid; r < k; r += blockDim.x) a += vec[r] * vec[r];
+    const double n = sqrt(block_sum(a, sh));
+    for (int r = tid; r < k; r += blockDim.x) W[int64_t(c) * J.ldw + r] = vec[r] / n;
+    if (tid == 0) nrm[c] = n;
+    __syncthreads();
+  }
+}
+
+void gram_vectors(hipStream_t st, double* base, const void* descs, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gram_vectors, dim3(n), dim3(kLargeThreads), 0, st, base,
+                     static_cast<const GramVec*>(descs));
+}
+
+}  // namespace kern
+}  // namespace qtt

@@ -83,7 +83,8 @@ struct Batched {
 };
 
 void launch_gemm(Context* ctx, double* ab, const std::vector<kern::GemmOut>& outs,
-                 const std::vector<kern::GemmPair>& pairs, const char* name) {
+                 const std::vector<kern::GemmPair>& pairs, const char* name, double alpha = 1.0,
+                 double beta = 0.0) {
   std::vector<kern::GemmTile> tiles;
   for (size_t o = 0; o < outs.size(); ++o) {
     const int tm = (outs[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
@@ -94,7 +95,7 @@ void launch_gemm(Context* ctx, double* ab, const std::vector<kern::GemmOut>& out
   if (tiles.empty()) return;
   KernelTimer kt(*ctx, name);
   kern::gemm_f64(ctx->stream(), ab, ctx->upload_vec(tiles), int(tiles.size()), ctx->upload_vec(outs),
-                 ctx->upload_vec(pairs), 1.0, 0.0);
+                 ctx->upload_vec(pairs), alpha, beta);
 }
 
 // Cyclic block Jacobi sweeps over the sectors listed in `which` until a sweep
@@ -282,37 +283,66 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   const size_t lds_margin = 256;
   auto lds_bytes = [](int64_t ell, int64_t k) { return size_t(ell * (k + (k & 1))) * sizeof(double); };
   const bool use_small = env_int("QTT_SVD_SMALL", 1) != 0;
-  const bool gram_multi = env_int("QTT_SVD_GRAM_SOLVER", 0) == 1;
   std::vector<char> done(ns, 0), big_only(ns, 0);
   std::vector<BufPtr> keep;  // Gram and ping-pong buffers live until the finalize is enqueued
 
-  // ---- Gram preconditioning of long sectors ---------------------------------
+  // ---- Gram iterations ----------------------------------------------------------
+  // Every SVD sector goes through G = Y^T Y, eigenvectors W_g of G, Y <- Y W_g,
+  // W <- W W_g until G is diagonal to the tolerance (relative, or absolute
+  // with respect to the largest singular value; the second case ends with
+  // two Gram-Schmidt passes on the short columns).  Sectors with k <=
+  // kGramEigMaxK solve for W_g in one work-group (LDS); wider ones run the
+  // pivoted Cholesky by panels, block Jacobi rounds on L = R^T and a
+  // normalisation pass, all with G in device memory.
+  const size_t pchol_limit = kern::pchol_lds_limit();
   std::vector<int> wide;
+  std::vector<char> large(ns, 0);
   for (int i = 0; i < ns; ++i) {
     const auto& s = B.secs[i];
-    const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
-    const bool long_enough = s.ell >= 2 * kp || env_int("QTT_SVD_GRAM_ALL", 1) != 0;
-    if (!eigh_mode && use_small && long_enough && s.k <= kern::kGramEigMaxK &&
-        (gram_multi || kern::gram_eig_lds_bytes(s.k) + lds_margin <= lds_limit))
+    if (eigh_mode || !use_small || s.k < 1) continue;
+    if (s.k <= kern::kGramEigMaxK && kern::gram_eig_lds_bytes(s.k) + lds_margin <= lds_limit) {
       wide.push_back(i);
+    } else if (size_t(s.k) * 4 * sizeof(double) <= pchol_limit && env_int("QTT_SVD_LARGE_GRAM", 1)) {
+      wide.push_back(i);
+      large[i] = 1;
+    }
   }
   if (!wide.empty()) {
     const int nw = int(wide.size());
+    int kmax_large = 0;
+    for (int i : wide)
+      if (large[i]) kmax_large = std::max(kmax_large, B.secs[i].k);
+    const int pnb = kmax_large ? int(std::min<size_t>(32, pchol_limit / (size_t(kmax_large) * sizeof(double)))) : 0;
     int64_t elems = 0;
-    std::vector<int64_t> go(nw), gwo(nw), y2o(nw), w2o(nw);
+    std::vector<int64_t> go(nw), gwo(nw), y2o(nw), w2o(nw), lo(nw, -1), lpo(nw, -1), wdo(nw, -1), sco(nw, -1),
+        sto(nw, -1);
     size_t lds_need = 0;
     for (int q = 0; q < nw; ++q) {
       const auto& s = B.secs[wide[q]];
       const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+      const int64_t k = s.k;
       go[q] = elems;
-      elems += int64_t(s.k) * (gram_multi ? kp : s.k);
+      elems += k * k;
       gwo[q] = elems;
-      elems += gram_multi ? kp * kp : int64_t(s.k) * s.k;
+      elems += k * k;
       y2o[q] = elems;
       elems += int64_t(s.ell) * kp;
       w2o[q] = elems;
       elems += wsz[wide[q]];
-      lds_need = std::max(lds_need, kern::gram_eig_lds_bytes(s.k));
+      if (large[wide[q]]) {
+        lo[q] = elems;  // L, k x kp with zero padding columns
+        elems += k * kp;
+        lpo[q] = elems;
+        elems += k * pnb;
+        wdo[q] = elems;  // rotation sink of the block Jacobi rounds
+        elems += kp * kp + 2 * kp + k + 1;
+        sco[q] = elems;
+        elems += 3 * k;
+        sto[q] = elems;  // ints
+        elems += (k + 2) / 2 + 1;
+      } else {
+        lds_need = std::max(lds_need, kern::gram_eig_lds_bytes(s.k));
+      }
     }
     BufPtr gbuf = ctx->alloc(size_t(elems) * sizeof(double));
     keep.push_back(gbuf);
@@ -324,16 +354,6 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       for (int q = 0; q < nw; ++q) {
         const auto& s = B.secs[wide[q]];
         const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
-        if (gram_multi) {  // padding columns of G
-          kern::Copy2D c{};
-          c.src_off = -1;
-          c.dst_off = gb + go[q] + int64_t(s.k) * s.k;
-          c.rows = 1;
-          c.cols = int32_t(int64_t(s.k) * (kp - s.k));
-          c.dst_cs = 1;
-          add_chunks(zt, int(z.size()), int64_t(c.cols));
-          z.push_back(c);
-        }
         kern::Copy2D c{};
         c.src_off = -1;
         c.dst_off = gb + y2o[q];
@@ -361,12 +381,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     std::vector<int32_t> h_flags(size_t(2 * nw), 0);
     std::vector<int> act(nw), corr;
     std::iota(act.begin(), act.end(), 0);
-    for (int it = 0; it < 6 && !act.empty(); ++it) {
-      // G = Y^T Y over the k real columns; its eigenvectors W_g in one work-group
+    auto gram_of = [&](const std::vector<int>& qs) {  // G = Y^T Y over the k real columns
       std::vector<kern::GemmOut> outs;
       std::vector<kern::GemmPair> prs;
-      std::vector<kern::SmallJacobi> jd;
-      for (int q : act) {
+      for (int q : qs) {
         const auto& s = B.secs[wide[q]];
         kern::GemmPair gp{};
         gp.a_off = s.y_off;
@@ -378,61 +396,35 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         gp.K = s.ell;
         outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
         prs.push_back(gp);
-        jd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, q, nw + q});
       }
       launch_gemm(ctx, ab, outs, prs, "svd_gram");
-      if (gram_multi) {
-        std::vector<kern::SvdSector> gsec;
-        std::vector<int> all;
-        for (int q : act) {
-          kern::SvdSector g = B.secs[wide[q]];
-          g.y_off = gb + go[q];
-          g.w_off = gb + gwo[q];
-          g.ell = g.k;
-          g.flag_idx = int32_t(gsec.size());
-          all.push_back(int(gsec.size()));
-          gsec.push_back(g);
+    };
+    auto read_flags = [&] {
+      QTT_HIP(hipMemcpyAsync(h_flags.data(), d_flags, size_t(2 * nw) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             ctx->stream()));
+      ctx->sync();
+    };
+    auto retire = [&](std::vector<int>& qs) {  // drop converged sectors after a flag read
+      std::vector<int> next;
+      for (int q : qs) {
+        if (h_flags[q] == 0) {
+          done[wide[q]] = 1;
+          if (h_flags[nw + q] > 0) corr.push_back(q);  // converged in the absolute sense only
+        } else {
+          next.push_back(q);
         }
-        {
-          KernelTimer kt(*ctx, "svd_init");
-          kern::svd_init_w(ctx->stream(), ab, ctx->upload_vec(gsec), int(gsec.size()));
-        }
-        std::vector<int> fr;
-        jacobi_sweeps(ctx, ab, gsec, all, tol, inner, max_sweeps, &fr);
-        for (size_t j = 0; j < act.size(); ++j) h_flags[act[j]] = fr[j];
-      } else {
-        static const char* const kEigNames[] = {"svd_eig", "svd_eig_it1", "svd_eig_it2", "svd_eig_it3+"};
-        KernelTimer kt(*ctx, kEigNames[std::min(it, 3)]);
-        kern::gram_eig(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), d_flags, tol, max_sweeps,
-                       env_int("QTT_SVD_PRECOND", 1), lds_need);
       }
-      if (it >= 1) {  // G diagonal to tol: the columns of Y are orthogonal, the sector is done
-        if (!gram_multi) {
-          QTT_HIP(hipMemcpyAsync(h_flags.data(), d_flags, size_t(2 * nw) * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                 ctx->stream()));
-          ctx->sync();
-        }
-        std::vector<int> next;
-        for (int q : act) {
-          if (h_flags[q] == 0) {
-            done[wide[q]] = 1;
-            if (h_flags[nw + q] > 0) corr.push_back(q);  // converged in the absolute sense only
-          } else {
-            next.push_back(q);
-          }
-        }
-        act.swap(next);
-        if (act.empty()) break;
-      }
-      // Y2^T = W_g^T Y^T and W2^T = W_g^T W^T (leading k x k block of W)
-      outs.clear();
-      prs.clear();
-      for (int q : act) {
+      qs.swap(next);
+    };
+    auto rotate = [&](const std::vector<int>& qs) {  // Y2^T = W_g^T Y^T, W2^T = W_g^T W^T
+      std::vector<kern::GemmOut> outs;
+      std::vector<kern::GemmPair> prs;
+      for (int q : qs) {
         const auto& s = B.secs[wide[q]];
         const int kp = s.nb * kern::kSvdBlock;
         kern::GemmPair p1{};
         p1.a_off = gb + gwo[q];
-        p1.a_rs = gram_multi ? kp : s.k;
+        p1.a_rs = s.k;
         p1.a_cs = 1;
         p1.b_off = s.y_off;
         p1.b_rs = s.ell;
@@ -447,11 +439,132 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         prs.push_back(p2);
       }
       launch_gemm(ctx, ab, outs, prs, "svd_rotate");
-      for (int q : act) {
+      for (int q : qs) {
         auto& s = B.secs[wide[q]];
         std::swap(s.y_off, alt_y[q]);
         std::swap(s.w_off, alt_w[q]);
       }
+    };
+    auto solve_large = [&](const std::vector<int>& qs) {  // W_g of wide sectors, G in device memory
+      if (qs.empty()) return;
+      {
+        std::vector<kern::Copy2D> z;
+        std::vector<kern::ChunkTask> zt;
+        std::vector<kern::SvdSector> wi;
+        for (int q : qs) {
+          const auto& s = B.secs[wide[q]];
+          const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+          kern::Copy2D c{};
+          c.src_off = -1;
+          c.dst_off = gb + lo[q];
+          c.rows = 1;
+          c.cols = int32_t(int64_t(s.k) * kp);
+          c.dst_cs = 1;
+          add_chunks(zt, int(z.size()), int64_t(c.cols));
+          z.push_back(c);
+          kern::SvdSector w = s;
+          w.w_off = gb + wdo[q];
+          wi.push_back(w);
+        }
+        KernelTimer kt(*ctx, "svd_init");
+        kern::copy2d(ctx->stream(), ab, ctx->upload_vec(z), ctx->upload_vec(zt), int(zt.size()));
+        kern::svd_init_w(ctx->stream(), ab, ctx->upload_vec(wi), int(wi.size()));
+        for (int q : qs) {
+          int32_t* st = reinterpret_cast<int32_t*>(ab + gb + sto[q]);
+          kern::fill_i32(ctx->stream(), st, B.secs[wide[q]].k + 1, 0);
+          kern::fill_i32(ctx->stream(), st, 1, B.secs[wide[q]].k);
+        }
+      }
+      int kact = 0;
+      for (int q : qs) kact = std::max(kact, B.secs[wide[q]].k);
+      for (int j0 = 0; j0 < kact; j0 += pnb) {
+        std::vector<kern::CholPanel> pd;
+        std::vector<kern::GemmOut> outs;
+        std::vector<kern::GemmPair> prs;
+        int kp_max = 0;
+        for (int q : qs) {
+          const auto& s = B.secs[wide[q]];
+          if (s.k <= j0) continue;
+          kp_max = std::max(kp_max, s.k);
+          pd.push_back({gb + go[q], gb + lo[q], gb + lpo[q], gb + sto[q], s.k, 0});
+          kern::GemmPair gp{};  // G -= Lp Lp^T
+          gp.a_off = gb + lpo[q];
+          gp.a_rs = 1;
+          gp.a_cs = s.k;
+          gp.b_off = gb + lpo[q];
+          gp.b_rs = s.k;
+          gp.b_cs = 1;
+          gp.K = pnb;
+          outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
+          prs.push_back(gp);
+        }
+        {
+          KernelTimer kt(*ctx, "svd_pchol");
+          kern::pchol_panel(ctx->stream(), ab, ctx->upload_vec(pd), int(pd.size()), j0, pnb, kp_max);
+        }
+        if (j0 + pnb < kact) launch_gemm(ctx, ab, outs, prs, "svd_pchol_update", -1.0, 1.0);
+      }
+      std::vector<kern::SvdSector> gsec;
+      std::vector<int> all;
+      std::vector<kern::GramVec> vd;
+      for (int q : qs) {
+        const auto& s = B.secs[wide[q]];
+        kern::SvdSector g = s;
+        g.y_off = gb + lo[q];
+        g.w_off = gb + wdo[q];
+        g.ell = s.k;
+        g.flag_idx = int32_t(gsec.size());
+        all.push_back(int(gsec.size()));
+        gsec.push_back(g);
+        vd.push_back({gb + lo[q], gb + gwo[q], gb + sco[q], s.k, s.k, s.k, 0});
+      }
+      jacobi_sweeps(ctx, ab, gsec, all, tol, inner, max_sweeps, nullptr);
+      KernelTimer kt(*ctx, "svd_vectors");
+      kern::gram_vectors(ctx->stream(), ab, ctx->upload_vec(vd), int(vd.size()));
+    };
+    const int max_it = env_int("QTT_SVD_GRAM_ITERS", 6);
+    for (int it = 0; it < max_it && !act.empty(); ++it) {
+      gram_of(act);
+      std::vector<int> act_s, act_l;
+      for (int q : act) (large[wide[q]] ? act_l : act_s).push_back(q);
+      // small sectors: test and solve in one work-group each
+      if (!act_s.empty()) {
+        std::vector<kern::SmallJacobi> jd;
+        for (int q : act_s) {
+          const auto& s = B.secs[wide[q]];
+          jd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, q, nw + q});
+        }
+        static const char* const kEigNames[] = {"svd_eig", "svd_eig_it1", "svd_eig_it2", "svd_eig_it3+"};
+        KernelTimer kt(*ctx, kEigNames[std::min(it, 3)]);
+        kern::gram_eig(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), d_flags, tol, max_sweeps,
+                       env_int("QTT_SVD_PRECOND", 1), lds_need);
+      }
+      // large sectors: test first (the solve is a long chain of launches); the
+      // flag read also covers the small solver launched above
+      int cd_launched = 0;
+      if (!act_l.empty()) {
+        cd_launched = 1;
+        std::vector<kern::SmallJacobi> cd;
+        for (int q : act_l) {
+          const auto& s = B.secs[wide[q]];
+          cd.push_back({gb + go[q], -1, s.k, s.k, s.k, s.k, q, nw + q});
+        }
+        {
+          KernelTimer kt(*ctx, "svd_check");
+          kern::gram_check(ctx->stream(), ab, ctx->upload_vec(cd), int(cd.size()), d_flags, tol);
+        }
+        read_flags();
+        retire(act_l);
+        solve_large(act_l);
+      }
+      if (it >= 1) {  // the small solver's test of this iteration's G
+        if (!act_s.empty() && act_l.empty() && cd_launched == 0) read_flags();
+        retire(act_s);
+      }
+      act = act_s;
+      act.insert(act.end(), act_l.begin(), act_l.end());
+      if (act.empty()) break;
+      rotate(act);
       B.sweeps += 1;
     }
     for (int q : act) big_only[wide[q]] = 1;  // keeps its accumulated W: multi-WG path only
@@ -460,26 +573,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     // make the short columns orthogonal; W is unchanged, and the dropped
     // triangular part is below tol times the largest singular value.
     for (int pass = 0; pass < 2 && !corr.empty(); ++pass) {
-      std::vector<kern::GemmOut> outs;
-      std::vector<kern::GemmPair> prs;
-      if (pass > 0) {  // the first pass reuses the last Gram matrix of the loop
-        for (int q : corr) {
-          const auto& s = B.secs[wide[q]];
-          kern::GemmPair gp{};
-          gp.a_off = s.y_off;
-          gp.a_rs = s.ell;
-          gp.a_cs = 1;
-          gp.b_off = s.y_off;
-          gp.b_rs = 1;
-          gp.b_cs = s.ell;
-          gp.K = s.ell;
-          outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
-          prs.push_back(gp);
-        }
-        launch_gemm(ctx, ab, outs, prs, "svd_gram");
-        outs.clear();
-        prs.clear();
-      }
+      if (pass > 0) gram_of(corr);  // the first pass reuses the last Gram matrix of the loop
       std::vector<kern::SmallJacobi> gd;
       for (int q : corr) {
         const auto& s = B.secs[wide[q]];
@@ -489,6 +583,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         KernelTimer kt(*ctx, "svd_gs");
         kern::gs_step(ctx->stream(), ab, ctx->upload_vec(gd), int(gd.size()));
       }
+      std::vector<kern::GemmOut> outs;
+      std::vector<kern::GemmPair> prs;
       for (int q : corr) {
         const auto& s = B.secs[wide[q]];
         kern::GemmPair p1{};

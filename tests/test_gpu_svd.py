@@ -184,3 +184,27 @@ def test_eigh_matches_dense(ctx):
     Ud = U.to_dense()
     dd = np.concatenate([d.blocks[(i,)] for i in range(len(d.secs[0]))])
     assert np.abs(A.to_dense() @ Ud - Ud * dd).max() <= 1e-11 * nrm
+
+
+@pytest.mark.parametrize("m,n,decades", [(300, 200, 12), (200, 300, 8), (260, 150, 3)])
+def test_svd_large_graded_sector(ctx, m, n, decades):
+    """One sector wider than the one-work-group eigen-solver takes (k > 140),
+    with a graded spectrum like a DMRG two-site tensor: singular values within
+    1e-12 of the largest against the dense SVD (library routine), U and V
+    orthonormal, reconstruction within 1e-12."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(m + n)
+    k = min(m, n)
+    U0, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, k)))
+    s = 10.0 ** (-decades * np.arange(k) / k)
+    A = BTensor(G1, [[m], [n]], [[(0,)], [(0,)]], (0,))
+    A.blocks[(0, 0)] = (U0 * s) @ V0.T
+    U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, A), 1))
+    dd = _check_svd(A, U, d, V, 1)
+    ref = np.linalg.svd(A.blocks[(0, 0)], compute_uv=False)
+    assert np.abs(np.sort(dd)[::-1] - ref).max() <= 1e-12 * ref[0]
+    Ud = U.to_dense()
+    Vd = V.to_dense()
+    assert np.abs(Ud.T @ Ud - np.eye(k)).max() < 1e-10
+    assert np.abs(Vd.T @ Vd - np.eye(k)).max() < 1e-12
