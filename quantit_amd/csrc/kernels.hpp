@@ -105,7 +105,7 @@ void normalize(hipStream_t s, double* base, int64_t off, int64_t n);
 // in place by block one-sided Jacobi; W (k x k, column-major) accumulates the
 // rotations.  See svd_jacobi.hip.
 struct SvdSector {
-  int64_t y_off, w_off;     // work matrices
+  int64_t y_off, w_off;     // work matrices (w_off < 0: no right rotation kept)
   int64_t u_off, v_off;     // outputs: U (m x k) and V (n x k) row-major
   int64_t s_off;            // singular values (k), descending
   int32_t m, n, ell, k;     // ell = max(m,n), k = min(m,n)

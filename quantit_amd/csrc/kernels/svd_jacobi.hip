@@ -28,7 +28,8 @@ namespace kern {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int NB2 = 2 * kSvdBlock;  // 32 columns per work-group
-constexpr int YP = 66;              // LDS pitch of the staged Y chunk
+constexpr int YC = 128;             // rows of Y staged per chunk
+constexpr int YP = YC + 2;          // LDS pitch of the staged Y chunk
 constexpr int GP = NB2 + 1;         // LDS pitch of G and Q
 
 __global__ void k_svd_init_w(double* __restrict__ base, const SvdSector* __restrict__ secs) {
@@ -51,45 +52,78 @@ __device__ __forceinline__ int circle_block(int pos, int round, int nb) {
   return pos == 0 ? 0 : 1 + ((pos - 1 + round) % (nb - 1));
 }
 
+// Reciprocal square root and reciprocal by the hardware estimates refined by
+// two Newton steps (each doubles the correct bits), well inside the rounding
+// of the rotations they feed; the division and square root the compiler
+// expands to correctly rounded sequences cost several times more latency.
+__device__ __forceinline__ double fast_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+__device__ __forceinline__ double fast_rcp(double x) {
+  double z = __builtin_amdgcn_rcp(x);
+  z = z * (2.0 - x * z);
+  return z * (2.0 - x * z);
+}
+
+// Jacobi rotation (c, s) annihilating the off-diagonal h/2 of the symmetric
+// 2x2 [[a, h/2], [h/2, b]] with |angle| <= pi/4, from d = b - a:
+// t = sign(d) h / (|d| + sqrt(d^2 + h^2)), c = 1/sqrt(1 + t^2), s = c t.
+__device__ __forceinline__ void jacobi_cs(double d, double h, double& c, double& sn) {
+  const double q2 = d * d + h * h;
+  const double r = q2 * fast_rsqrt(q2);
+  const double t = (d >= 0.0 ? h : -h) * fast_rcp(fabs(d) + r);
+  c = fast_rsqrt(1.0 + t * t);
+  sn = c * t;
+}
+
 // Apply the 32x32 rotation Q (LDS, pitch GP) to the 32 selected columns of a
-// column-major matrix M (rows x ld) in place, 64 rows at a time.
+// column-major matrix M (rows x ld) in place, YC rows at a time (each wave
+// owns two 16-row MFMA tiles of a chunk).
 __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, const int* cols,
                                double (*Ys)[YP], const double (*Q)[GP]) {
+  constexpr int PER = YC / 8;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c = tid >> 3, lq = (tid & 7) * 8;
+  const int c = tid >> 3, lq = (tid & 7) * PER;
   const double* src = M + int64_t(cols[c]) * ld;
-  double buf[8];
+  double buf[PER];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) buf[i] = (lq + i < rows) ? src[lq + i] : 0.0;
-  for (int l0 = 0; l0 < rows; l0 += 64) {
+  for (int i = 0; i < PER; ++i) buf[i] = (lq + i < rows) ? src[lq + i] : 0.0;
+  for (int l0 = 0; l0 < rows; l0 += YC) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) Ys[c][lq + i] = buf[i];
+    for (int i = 0; i < PER; ++i) Ys[c][lq + i] = buf[i];
     __syncthreads();
-    if (l0 + 64 < rows) {
+    if (l0 + YC < rows) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int l = l0 + 64 + lq + i;
+      for (int i = 0; i < PER; ++i) {
+        const int l = l0 + YC + lq + i;
         buf[i] = (l < rows) ? src[l] : 0.0;
       }
     }
-    v4d acc[2] = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
+    v4d acc[2][2] = {{v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}},
+                     {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}}};
 #pragma unroll
     for (int kk = 0; kk < NB2; kk += 4) {
       const int kr = kk + (lane >> 4), cl = lane & 15;
-      const double a = Ys[kr][wave * 16 + cl];
+      const double a0 = Ys[kr][wave * 32 + cl], a1 = Ys[kr][wave * 32 + 16 + cl];
       const double b0 = Q[kr][cl], b1 = Q[kr][16 + cl];
-      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[1], 0, 0, 0);
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
     __syncthreads();  // every wave has read its Ys rows before they are overwritten
 #pragma unroll
-    for (int jt = 0; jt < 2; ++jt)
+    for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = l0 + wave * 16 + 4 * r + (lane >> 4);  // f64 MFMA output layout
-        const int col = jt * 16 + (lane & 15);
-        if (row < rows) M[int64_t(cols[col]) * ld + row] = acc[jt][r];
-      }
+      for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = l0 + wave * 32 + rt * 16 + 4 * r + (lane >> 4);  // f64 MFMA output layout
+          const int col = jt * 16 + (lane & 15);
+          if (row < rows) M[int64_t(cols[col]) * ld + row] = acc[rt][jt][r];
+        }
   }
 }
 
@@ -99,8 +133,6 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   __shared__ double Ys[NB2][YP];
   __shared__ double G[2][NB2][GP];
   __shared__ double Q[2][NB2][GP];
-  __shared__ double cc[NB2], sg[NB2];
-  __shared__ int partner[NB2];
   __shared__ int cols[NB2];
   __shared__ int flag, nrot;
 
@@ -122,26 +154,27 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   const int wi = wave >> 1, wj = wave & 1;
   v4d g = v4d{0.0, 0.0, 0.0, 0.0};
   {
-    // 64-row chunks staged through LDS; the next chunk's global loads are
+    // YC-row chunks staged through LDS; the next chunk's global loads are
     // issued before the MFMAs of the current one
-    const int c = tid >> 3, lq = (tid & 7) * 8;
+    constexpr int PER = YC / 8;
+    const int c = tid >> 3, lq = (tid & 7) * PER;
     const double* src = Y + int64_t(cols[c]) * ld;
-    double buf[8];
+    double buf[PER];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) buf[i] = (lq + i < ell) ? src[lq + i] : 0.0;
-    for (int l0 = 0; l0 < ell; l0 += 64) {
+    for (int i = 0; i < PER; ++i) buf[i] = (lq + i < ell) ? src[lq + i] : 0.0;
+    for (int l0 = 0; l0 < ell; l0 += YC) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) Ys[c][lq + i] = buf[i];
+      for (int i = 0; i < PER; ++i) Ys[c][lq + i] = buf[i];
       __syncthreads();
-      if (l0 + 64 < ell) {
+      if (l0 + YC < ell) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int l = l0 + 64 + lq + i;
+        for (int i = 0; i < PER; ++i) {
+          const int l = l0 + YC + lq + i;
           buf[i] = (l < ell) ? src[l] : 0.0;
         }
       }
 #pragma unroll
-      for (int kk = 0; kk < 64; kk += 4) {
+      for (int kk = 0; kk < YC; kk += 4) {
         const int kr = kk + (lane >> 4), cl = lane & 15;
         g = __builtin_amdgcn_mfma_f64_16x16x4f64(Ys[wi * 16 + cl][kr], Ys[wj * 16 + cl][kr], g, 0, 0, 0);
       }
@@ -162,54 +195,65 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   __syncthreads();
   if (flag == 0) return;
   if (tid == 0) atomicAdd(&counters[s.flag_idx], 1);
-  // ---- 3. eigen-decomposition of G by cyclic Jacobi --------------------------
+  // ---- 3. eigen-decomposition of G by cyclic two-sided Jacobi -----------------
+  // Thread (P, R) owns the 2x2 block of G at the row pair P and column pair R
+  // of the round (circle ordering, 16 disjoint pairs): it computes both
+  // pairs' rotations from the diagonal blocks, then applies J_P^T (rows) and
+  // J_R (columns) to its block and J_R to the same rows of Q, in place.  The
+  // 256 blocks tile G and Q, so one round is two barriers and no other LDS
+  // traffic.
   for (int e = tid; e < NB2 * NB2; e += blockDim.x) {
     const int i = e / NB2, j = e % NB2;
     Q[0][i][j] = (i == j) ? 1.0 : 0.0;
   }
-  int cur = 0;
   __syncthreads();
-  for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
-    if (tid == 0) nrot = 0;
-    __syncthreads();
-    for (int r = 0; r < NB2 - 1; ++r) {
-      if (tid < NB2 / 2) {
-        const int a = circle_block(tid, r, NB2), b = circle_block(NB2 - 1 - tid, r, NB2);
-        const int pp = min(a, b), qq = max(a, b);
-        const double app = G[cur][pp][pp], aqq = G[cur][qq][qq], apq = G[cur][pp][qq];
-        double c = 1.0, sn = 0.0;
-        if (app > 0.0 && aqq > 0.0 && fabs(apq) > DBL_EPSILON * sqrt(app) * sqrt(aqq)) {
-          const double tau = (aqq - app) / (2.0 * apq);
-          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          c = 1.0 / sqrt(1.0 + t * t);
-          sn = c * t;
-          atomicAdd(&nrot, 1);
-        }
-        partner[pp] = qq;
-        partner[qq] = pp;
-        cc[pp] = c;
-        cc[qq] = c;
-        sg[pp] = -sn;
-        sg[qq] = sn;
+  {
+    const int P = tid >> 4, R = tid & 15;
+    auto rot = [&](int x, int y, double& c, double& sn) {
+      const double app = G[0][x][x], aqq = G[0][y][y], apq = G[0][x][y];
+      c = 1.0;
+      sn = 0.0;
+      if (app > 0.0 && aqq > 0.0 && apq * apq > (DBL_EPSILON * app) * (DBL_EPSILON * aqq)) {
+        jacobi_cs(aqq - app, 2.0 * apq, c, sn);
+        return true;
       }
+      return false;
+    };
+    for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
+      if (tid == 0) nrot = 0;
       __syncthreads();
-      const int nx = cur ^ 1;
-      for (int e = tid; e < NB2 * NB2; e += blockDim.x) {
-        const int a = e / NB2, b = e % NB2;
-        const int a2 = partner[a], b2 = partner[b];
-        const double ca = cc[a], sa = sg[a], cb = cc[b], sb = sg[b];
-        G[nx][a][b] = ca * cb * G[cur][a][b] + ca * sb * G[cur][a][b2] + sa * cb * G[cur][a2][b] +
-                      sa * sb * G[cur][a2][b2];
-        Q[nx][a][b] = cb * Q[cur][a][b] + sb * Q[cur][a][b2];
+      for (int r = 0; r < NB2 - 1; ++r) {
+        const int p0 = circle_block(P, r, NB2), p1 = circle_block(NB2 - 1 - P, r, NB2);
+        const int q0 = circle_block(R, r, NB2), q1 = circle_block(NB2 - 1 - R, r, NB2);
+        const int a = min(p0, p1), a2 = max(p0, p1), b = min(q0, q1), b2 = max(q0, q1);
+        double cp, sp, cq, sq;
+        const bool rp = rot(a, a2, cp, sp);
+        rot(b, b2, cq, sq);
+        const double g00 = G[0][a][b], g01 = G[0][a][b2], g10 = G[0][a2][b], g11 = G[0][a2][b2];
+        const double q00 = Q[0][a][b], q01 = Q[0][a][b2], q10 = Q[0][a2][b], q11 = Q[0][a2][b2];
+        __syncthreads();  // every read of this round (diagonal blocks included) precedes the writes
+        // rows a, a2 by J_P^T; columns b, b2 by J_R (row' a = c a - s a2, row' a2 = c a2 + s a)
+        const double h00 = cp * g00 - sp * g10, h01 = cp * g01 - sp * g11;
+        const double h10 = cp * g10 + sp * g00, h11 = cp * g11 + sp * g01;
+        G[0][a][b] = cq * h00 - sq * h01;
+        G[0][a][b2] = cq * h01 + sq * h00;
+        G[0][a2][b] = cq * h10 - sq * h11;
+        G[0][a2][b2] = cq * h11 + sq * h10;
+        Q[0][a][b] = cq * q00 - sq * q01;
+        Q[0][a][b2] = cq * q01 + sq * q00;
+        Q[0][a2][b] = cq * q10 - sq * q11;
+        Q[0][a2][b2] = cq * q11 + sq * q10;
+        if (R == 0 && rp) atomicAdd(&nrot, 1);
+        __syncthreads();
       }
-      cur = nx;
+      if (nrot == 0) break;
       __syncthreads();
     }
-    if (nrot == 0) break;
-    __syncthreads();
   }
+  const int cur = 0;
   // ---- 4. rotate the columns of Y and W ---------------------------------------
   apply_rotation(Y, ld, ell, cols, Ys, Q[cur]);
+  if (s.w_off < 0) return;  // caller keeps no right rotation
   __syncthreads();
   const int kp = nb * kSvdBlock;
   apply_rotation(base + s.w_off, kp, kp, cols, Ys, Q[cur]);
@@ -607,11 +651,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
         }
         group_sum3(al, be, ga, gsz);
         if (al > 0.0 && be > 0.0 && ga * ga > (tol_rot * al) * (tol_rot * be)) {
-          // t = tan(theta) of the rotation annihilating ga, |theta| <= pi/4:
-          // t = sign(d) 2 ga / (|d| + sqrt(d^2 + 4 ga^2)), d = be - al
-          const double d = be - al;
-          const double t = (d >= 0.0 ? 2.0 * ga : -2.0 * ga) / (fabs(d) + sqrt(d * d + 4.0 * ga * ga));
-          const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+          double c, sn;  // rotation annihilating ga
+          jacobi_cs(be - al, 2.0 * ga, c, sn);
 #pragma unroll
           for (int j = 0; j < kEigR; ++j) {
             const int l = li + j * gsz;

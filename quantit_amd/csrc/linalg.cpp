@@ -314,8 +314,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       if (large[i]) kmax_large = std::max(kmax_large, B.secs[i].k);
     const int pnb = kmax_large ? int(std::min<size_t>(32, pchol_limit / (size_t(kmax_large) * sizeof(double)))) : 0;
     int64_t elems = 0;
-    std::vector<int64_t> go(nw), gwo(nw), y2o(nw), w2o(nw), lo(nw, -1), lpo(nw, -1), wdo(nw, -1), sco(nw, -1),
-        sto(nw, -1);
+    std::vector<int64_t> go(nw), gwo(nw), y2o(nw), w2o(nw), lo(nw, -1), lpo(nw, -1), sco(nw, -1), sto(nw, -1);
     size_t lds_need = 0;
     for (int q = 0; q < nw; ++q) {
       const auto& s = B.secs[wide[q]];
@@ -334,8 +333,6 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         elems += k * kp;
         lpo[q] = elems;
         elems += k * pnb;
-        wdo[q] = elems;  // rotation sink of the block Jacobi rounds
-        elems += kp * kp + 2 * kp + k + 1;
         sco[q] = elems;
         elems += 3 * k;
         sto[q] = elems;  // ints
@@ -450,7 +447,6 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       {
         std::vector<kern::Copy2D> z;
         std::vector<kern::ChunkTask> zt;
-        std::vector<kern::SvdSector> wi;
         for (int q : qs) {
           const auto& s = B.secs[wide[q]];
           const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
@@ -462,13 +458,9 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
           c.dst_cs = 1;
           add_chunks(zt, int(z.size()), int64_t(c.cols));
           z.push_back(c);
-          kern::SvdSector w = s;
-          w.w_off = gb + wdo[q];
-          wi.push_back(w);
         }
         KernelTimer kt(*ctx, "svd_init");
         kern::copy2d(ctx->stream(), ab, ctx->upload_vec(z), ctx->upload_vec(zt), int(zt.size()));
-        kern::svd_init_w(ctx->stream(), ab, ctx->upload_vec(wi), int(wi.size()));
         for (int q : qs) {
           int32_t* st = reinterpret_cast<int32_t*>(ab + gb + sto[q]);
           kern::fill_i32(ctx->stream(), st, B.secs[wide[q]].k + 1, 0);
@@ -511,7 +503,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         const auto& s = B.secs[wide[q]];
         kern::SvdSector g = s;
         g.y_off = gb + lo[q];
-        g.w_off = gb + wdo[q];
+        g.w_off = -1;  // the eigenvectors come from the normalised columns
         g.ell = s.k;
         g.flag_idx = int32_t(gsec.size());
         all.push_back(int(gsec.size()));
