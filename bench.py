@@ -389,6 +389,58 @@ def grouped_cpu_baseline(A, B, S, budget):
             "gemm_tflops": g_rate / 1e12, "svd_model_tflops": sf / ss / 1e12}
 
 
+def bench_serving(args):
+    """Seeded Poisson trace of XXZ ground-state requests served by solve
+    slots with admission by HBM footprint: p50/p99 latency and solves/s."""
+    from quantit_amd import serving
+    if args.full_trace:
+        L_range, D_choices = (16, 128), (64, 128, 256, 512)
+    else:  # bounded default: a run of a few minutes
+        L_range, D_choices = (16, 64), (64, 128, 256)
+    trace = serving.make_trace(args.requests, rate=args.rate, seed=5, L_range=L_range, D_choices=D_choices)
+    srv = serving.Server(n_slots=args.slots, hbm_budget=int(args.hbm_gib * (1 << 30)),
+                         max_sweeps=args.sweeps or 20)
+
+    def progress(r, x):
+        msg = (f"request {r.rid}: L={r.L} D={r.D} delta={r.delta:.3f} "
+               + (f"E={x.energy:.12f} sweeps={x.sweeps} latency={x.latency:.2f}s" if isinstance(x, serving.Result)
+                  else f"error {x}"))
+        print(msg, file=sys.stderr, flush=True)
+
+    res, wall = srv.run(trace, progress=progress)
+    summ = serving.summarize(res, wall)
+    ok = [r for r in res if isinstance(r, serving.Result)]
+    out = {
+        "metric": "DMRG sweep s & contraction TFLOP/s @|dE/E|<=1e-10; served solves/s + p99 latency",
+        "value": summ["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(trace),
+        "warmup": 0,
+        "ms_per_step": 1e3 * wall / max(1, len(ok)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {len(trace)} XXZ chain requests, Poisson rate {args.rate}/s (seed 5), "
+                               f"L ~ U{{{L_range[0]}..{L_range[1]}}}, D ~ U{set(D_choices)}, delta ~ U[0.5,1.5], "
+                               f"Sz=0, |dE/E| <= 1e-10 or {args.sweeps or 20} sweeps",
+                   "slots": args.slots, "hbm_budget_gib": args.hbm_gib},
+        "p50_latency_s": summ["p50_latency_s"],
+        "p99_latency_s": summ["p99_latency_s"],
+        "failed": summ["failed"],
+        "mean_sweeps": summ["mean_sweeps"],
+        "requests": [{"L": r.L, "D": r.D, "delta": round(r.delta, 3), "latency_s": round(x.latency, 3),
+                      "solve_s": round(x.finish - x.start, 3), "sweeps": x.sweeps, "energy": x.energy}
+                     if isinstance(x, serving.Result) else {"L": r.L, "D": r.D, "error": str(x)}
+                     for r, x in zip(trace, res)],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -401,7 +453,13 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the split-FP32 side measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--workload", choices=["contract_svd_D1024", "grouped_batch"] + sorted(DMRG_WORKLOADS),
+    ap.add_argument("--requests", type=int, default=8, help="serving: trace length")
+    ap.add_argument("--rate", type=float, default=4.0, help="serving: Poisson arrival rate (1/s)")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent solve slots")
+    ap.add_argument("--hbm-gib", type=float, default=160.0, help="serving: HBM budget for admission")
+    ap.add_argument("--full-trace", action="store_true",
+                    help="serving: L up to 128 and D up to 512 (long run) instead of L <= 64, D <= 256")
+    ap.add_argument("--workload", choices=["contract_svd_D1024", "grouped_batch", "serving"] + sorted(DMRG_WORKLOADS),
                     default="contract_svd_D1024")
     ap.add_argument("--sweeps", type=int, default=0, help="DMRG workloads: sweeps (0 = the workload's own)")
     args = ap.parse_args()
@@ -416,6 +474,9 @@ def main():
         return
     if args.workload == "grouped_batch":
         bench_grouped(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
 
     import quantit_amd as qa
