@@ -17,6 +17,7 @@
 // columns, so the small ones keep absolute accuracy eps*||A||), which matches
 // the accuracy class of LAPACK gesdd used by the reference
 // (sources/btensor_linalg.cpp:483).
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 
@@ -127,14 +128,21 @@ __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, con
   }
 }
 
+// RESIDENT: the block pair's ell x 32 columns are loaded into dynamic LDS
+// once (ell <= the LDS capacity), the Gram matrix and the rotated columns are
+// computed from there and written back once; otherwise 128-row chunks are
+// staged twice (Gram, then rotation).
+template <bool RESIDENT>
 __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, const SvdSector* __restrict__ secs,
                                                    const int32_t* __restrict__ work, int round,
-                                                   int* __restrict__ counters, double tol, int inner_sweeps) {
-  __shared__ double Ys[NB2][YP];
-  __shared__ double G[2][NB2][GP];
-  __shared__ double Q[2][NB2][GP];
+                                                   int* __restrict__ counters, double tol, int inner_sweeps,
+                                                   int pitch) {
+  extern __shared__ double dyn[];
+  __shared__ double G[1][NB2][GP];
+  __shared__ double Q[1][NB2][GP];
   __shared__ int cols[NB2];
   __shared__ int flag, nrot;
+  double (*Ys)[YP] = reinterpret_cast<double (*)[YP]>(dyn);  // chunk staging (aliases the resident block)
 
   const SvdSector s = secs[work[2 * blockIdx.x]];
   const int pi = work[2 * blockIdx.x + 1];
@@ -153,7 +161,33 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   // ---- 1. Gram matrix ------------------------------------------------------
   const int wi = wave >> 1, wj = wave & 1;
   v4d g = v4d{0.0, 0.0, 0.0, 0.0};
-  {
+  if (RESIDENT) {
+    // all 32 columns at once: every load in flight together
+    double* Yl = dyn;  // Yl[c * pitch + r]
+    constexpr int BATCH = 16;  // loads in flight per thread
+    const int total = NB2 * ell;
+    for (int e0 = tid; e0 < total; e0 += BATCH * 256) {
+      double v[BATCH];
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) {
+        const int e = e0 + u * 256;
+        v[u] = e < total ? Y[int64_t(cols[e / ell]) * ld + e % ell] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) {
+        const int e = e0 + u * 256;
+        if (e < total) Yl[(e / ell) * pitch + e % ell] = v[u];
+      }
+    }
+    __syncthreads();
+    const int cl = lane & 15, kq = lane >> 4;
+    for (int kk = 0; kk < ell; kk += 4) {
+      const int kr = kk + kq;
+      const double a = kr < ell ? Yl[(wi * 16 + cl) * pitch + kr] : 0.0;
+      const double b = kr < ell ? Yl[(wj * 16 + cl) * pitch + kr] : 0.0;
+      g = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, g, 0, 0, 0);
+    }
+  } else {
     // YC-row chunks staged through LDS; the next chunk's global loads are
     // issued before the MFMAs of the current one
     constexpr int PER = YC / 8;
@@ -252,18 +286,75 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   }
   const int cur = 0;
   // ---- 4. rotate the columns of Y and W ---------------------------------------
-  apply_rotation(Y, ld, ell, cols, Ys, Q[cur]);
+  if (RESIDENT) {
+    const double* Yl = dyn;
+    for (int l0 = 0; l0 < ell; l0 += 4 * 32) {
+      v4d acc[2][2] = {{v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}},
+                       {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}}};
+      const int cl = lane & 15;
+#pragma unroll
+      for (int kk = 0; kk < NB2; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        const int r0 = l0 + wave * 32 + cl, r1 = r0 + 16;
+        const double a0 = r0 < ell ? Yl[kr * pitch + r0] : 0.0;
+        const double a1 = r1 < ell ? Yl[kr * pitch + r1] : 0.0;
+        const double b0 = Q[cur][kr][cl], b1 = Q[cur][kr][16 + cl];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = l0 + wave * 32 + rt * 16 + 4 * r + (lane >> 4);  // f64 MFMA output layout
+            const int col = jt * 16 + (lane & 15);
+            if (row < ell) Y[int64_t(cols[col]) * ld + row] = acc[rt][jt][r];
+          }
+    }
+  } else {
+    apply_rotation(Y, ld, ell, cols, Ys, Q[cur]);
+  }
   if (s.w_off < 0) return;  // caller keeps no right rotation
-  __syncthreads();
+  __syncthreads();           // the staging area below aliases the resident block
   const int kp = nb * kSvdBlock;
   apply_rotation(base + s.w_off, kp, kp, cols, Ys, Q[cur]);
 }
 
+// Rows of a block pair that stay resident in LDS (dynamic size at most the
+// device's per-block LDS less the static arrays).
+int svd_round_resident_rows() {
+  static int rows = -1;
+  if (rows < 0) {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    const int dyn_max = v - int(2 * NB2 * GP * sizeof(double)) - 1024;
+    rows = dyn_max / int(NB2 * sizeof(double)) - 1;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_svd_round<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_svd_round<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+  }
+  return rows;
+}
+
 void svd_round(hipStream_t st, double* base, const SvdSector* secs, const int32_t* work, int nwork, int round,
-               int* counters, double tol, int inner_sweeps) {
+               int* counters, double tol, int inner_sweeps, int max_ell) {
   if (nwork <= 0) return;
-  hipLaunchKernelGGL(k_svd_round, dim3(nwork), dim3(256), 0, st, base, secs, work, round, counters, tol,
-                     inner_sweeps);
+  const size_t staging = size_t(NB2) * YP * sizeof(double);
+  if (max_ell <= svd_round_resident_rows()) {
+    const int pitch = max_ell + 1;
+    const size_t lds = std::max(size_t(NB2) * pitch * sizeof(double), staging);
+    hipLaunchKernelGGL(k_svd_round<true>, dim3(nwork), dim3(256), lds, st, base, secs, work, round, counters, tol,
+                       inner_sweeps, pitch);
+  } else {
+    hipLaunchKernelGGL(k_svd_round<false>, dim3(nwork), dim3(256), staging, st, base, secs, work, round, counters,
+                       tol, inner_sweeps, 0);
+  }
 }
 
 // Block-wide reductions (fixed order, deterministic).

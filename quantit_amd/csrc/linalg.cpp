@@ -116,11 +116,12 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   int sweeps = 0;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     std::vector<int32_t> work;
-    int max_nb = 0;
+    int max_nb = 0, max_ell = 0;
     for (int i = 0; i < ns; ++i) {
       if (!active[i]) continue;
       const int nb = secs[i].nb;
       max_nb = std::max(max_nb, nb);
+      max_ell = std::max(max_ell, secs[i].ell);
       for (int pi = 0; pi < nb / 2; ++pi) {
         work.push_back(i);
         work.push_back(pi);
@@ -131,7 +132,7 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     auto* d_work = ctx->upload_vec(work);
     for (int r = 0; r < max_nb - 1; ++r) {
       KernelTimer kt(*ctx, "svd_round");
-      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner);
+      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner, max_ell);
     }
     QTT_HIP(hipMemcpyAsync(h_cnt.data(), d_cnt, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
                            ctx->stream()));
