@@ -66,6 +66,12 @@ struct Copy2D {
   double alpha;
 };
 void copy2d(hipStream_t s, double* base, const Copy2D* descs, const ChunkTask* tasks, int ntasks);
+struct SumParts {
+  int64_t dst_off, src_off, n;
+  int32_t parts, pad;
+};
+// dst = sum of `parts` consecutive arrays of n doubles at src (fixed order).
+void sum_parts(hipStream_t s, double* base, const SumParts* descs, const ChunkTask* tasks, int ntasks);
 
 // ---- elementwise / reductions ---------------------------------------------
 // out = x (op) y with y broadcast: mode 0 y scalar, 1 y vector along the last

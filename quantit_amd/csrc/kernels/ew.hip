@@ -53,6 +53,26 @@ __global__ __launch_bounds__(256) void k_copy2d(double* __restrict__ base, const
   }
 }
 
+// dst[i] = sum over s < parts of src[s * n + i] (fixed order): the reduction of
+// split-K partial products.  desc = {dst_off, src_off, n, parts}.
+__global__ __launch_bounds__(256) void k_sum_parts(double* __restrict__ base, const SumParts* __restrict__ descs,
+                                                   const ChunkTask* __restrict__ tasks) {
+  const ChunkTask t = tasks[blockIdx.x];
+  const SumParts d = descs[t.desc];
+  const int64_t i0 = int64_t(t.chunk) * kChunk;
+  const int64_t i1 = i0 + kChunk < d.n ? i0 + kChunk : d.n;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double v = 0.0;
+    for (int p = 0; p < d.parts; ++p) v += base[d.src_off + p * d.n + i];
+    base[d.dst_off + i] = v;
+  }
+}
+
+void sum_parts(hipStream_t s, double* base, const SumParts* descs, const ChunkTask* tasks, int ntasks) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_sum_parts, dim3(ntasks), dim3(256), 0, s, base, descs, tasks);
+}
+
 void copy2d(hipStream_t s, double* base, const Copy2D* descs, const ChunkTask* tasks, int ntasks) {
   if (ntasks <= 0) return;
   hipLaunchKernelGGL(k_copy2d, dim3(ntasks), dim3(256), 0, s, base, descs, tasks);

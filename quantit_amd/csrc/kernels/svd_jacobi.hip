@@ -901,25 +901,44 @@ size_t jacobi_small_lds_limit() {
 
 // One work-group per sector: singular values (column norms of Y), their rank
 // in descending order (ties broken by column index) and the sorted values.
-__global__ __launch_bounds__(256) void k_svd_norms(double* __restrict__ base, const SvdSector* __restrict__ secs) {
+// Column norms of Y: grid (sector, chunk of 16 columns); each wave sums
+// four columns at once (four independent loads per row step).
+__global__ __launch_bounds__(256) void k_svd_colnorms(double* __restrict__ base, const SvdSector* __restrict__ secs) {
   const SvdSector s = secs[blockIdx.x];
   const int k = s.k, ell = s.ell, kp = s.nb * kSvdBlock;
+  const int c0 = blockIdx.y * 16 + (threadIdx.x >> 6) * 4;
+  if (c0 >= k) return;
   const double* Y = base + s.y_off;
+  double* sig = base + s.w_off + int64_t(kp) * kp;
+  const int lane = threadIdx.x & 63;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int l = lane; l < ell; l += 64) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (c0 + c < k) {
+        const double x = Y[int64_t(c0 + c) * ell + l];
+        acc[c] += x * x;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double a = acc[c];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    if (lane == 0 && c0 + c < k) sig[c0 + c] = sqrt(a);
+  }
+}
+
+// One work-group per sector: rank of every singular value in descending order
+// (ties broken by column index) and the sorted values.
+__global__ __launch_bounds__(256) void k_svd_norms(double* __restrict__ base, const SvdSector* __restrict__ secs) {
+  const SvdSector s = secs[blockIdx.x];
+  const int k = s.k, kp = s.nb * kSvdBlock;
   double* aux = base + s.w_off + int64_t(kp) * kp;
   double* sig = aux;
   double* rnk = aux + kp;
   double* S = base + s.s_off;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int j = wave; j < k; j += 4) {
-    double acc = 0.0;
-    for (int l = lane; l < ell; l += 64) {
-      const double x = Y[int64_t(j) * ell + l];
-      acc += x * x;
-    }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) sig[j] = sqrt(acc);
-  }
-  __syncthreads();
+  const int tid = threadIdx.x;
   for (int j = tid; j < k; j += blockDim.x) {
     const double sj = sig[j];
     int r = 0;
@@ -1041,7 +1060,9 @@ __global__ __launch_bounds__(256) void k_svd_complete(double* __restrict__ base,
 void svd_finalize(hipStream_t st, double* base, const SvdSector* secs, const SvdSector* host_secs, int nsec,
                   const int4* tasks, int ntasks) {
   if (nsec <= 0) return;
-  (void)host_secs;
+  int kmax = 1;
+  for (int i = 0; i < nsec; ++i) kmax = std::max(kmax, host_secs[i].k);
+  hipLaunchKernelGGL(k_svd_colnorms, dim3(nsec, (kmax + 15) / 16), dim3(256), 0, st, base, secs);
   hipLaunchKernelGGL(k_svd_norms, dim3(nsec), dim3(256), 0, st, base, secs);
   if (ntasks > 0) hipLaunchKernelGGL(k_svd_write, dim3(ntasks), dim3(256), 0, st, base, secs, tasks);
   hipLaunchKernelGGL(k_svd_complete, dim3(nsec), dim3(256), 0, st, base, secs);

@@ -380,22 +380,52 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     std::vector<int> act(nw), corr;
     std::iota(act.begin(), act.end(), 0);
     auto gram_of = [&](const std::vector<int>& qs) {  // G = Y^T Y over the k real columns
+      // Long sectors split K over several tiles (partials summed in a fixed
+      // order afterwards) so that few small tiles do not serialise ell rows.
       std::vector<kern::GemmOut> outs;
       std::vector<kern::GemmPair> prs;
-      for (int q : qs) {
+      std::vector<int> parts(qs.size(), 1);
+      int64_t pelems = 0;
+      for (size_t j = 0; j < qs.size(); ++j) {
+        const auto& s = B.secs[wide[qs[j]]];
+        parts[j] = int(std::min<int64_t>(8, std::max<int64_t>(1, s.ell / 512)));
+        if (parts[j] > 1) pelems += int64_t(parts[j]) * s.k * s.k;
+      }
+      BufPtr pbuf = pelems ? ctx->alloc(size_t(pelems) * sizeof(double)) : nullptr;
+      const int64_t pb = pbuf ? pbuf->dptr() - ab : 0;
+      std::vector<kern::SumParts> sums;
+      std::vector<kern::ChunkTask> st;
+      int64_t pcur = pb;
+      for (size_t j = 0; j < qs.size(); ++j) {
+        const int q = qs[j];
         const auto& s = B.secs[wide[q]];
-        kern::GemmPair gp{};
-        gp.a_off = s.y_off;
-        gp.a_rs = s.ell;
-        gp.a_cs = 1;
-        gp.b_off = s.y_off;
-        gp.b_rs = 1;
-        gp.b_cs = s.ell;
-        gp.K = s.ell;
-        outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
-        prs.push_back(gp);
+        const int np = parts[j];
+        const int64_t kc = (int64_t(s.ell) + np - 1) / np;
+        for (int p = 0; p < np; ++p) {
+          const int64_t k0 = p * kc, k1 = std::min<int64_t>(s.ell, k0 + kc);
+          kern::GemmPair gp{};
+          gp.a_off = s.y_off + k0;
+          gp.a_rs = s.ell;
+          gp.a_cs = 1;
+          gp.b_off = s.y_off + k0;
+          gp.b_rs = 1;
+          gp.b_cs = s.ell;
+          gp.K = int32_t(std::max<int64_t>(0, k1 - k0));
+          const int64_t c_off = np > 1 ? pcur + int64_t(p) * s.k * s.k : gb + go[q];
+          outs.push_back({c_off, s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
+          prs.push_back(gp);
+        }
+        if (np > 1) {
+          add_chunks(st, int(sums.size()), int64_t(s.k) * s.k);
+          sums.push_back({gb + go[q], pcur, int64_t(s.k) * s.k, np, 0});
+          pcur += int64_t(np) * s.k * s.k;
+        }
       }
       launch_gemm(ctx, ab, outs, prs, "svd_gram");
+      if (!st.empty()) {
+        KernelTimer kt(*ctx, "svd_gram_sum");
+        kern::sum_parts(ctx->stream(), ab, ctx->upload_vec(sums), ctx->upload_vec(st), int(st.size()));
+      }
     };
     auto read_flags = [&] {
       QTT_HIP(hipMemcpyAsync(h_flags.data(), d_flags, size_t(2 * nw) * sizeof(int32_t), hipMemcpyDeviceToHost,
