@@ -35,6 +35,30 @@ _SZ = np.diag([0.5, -0.5])
 _ID = np.eye(2)
 
 
+def read_mpo_text(text):
+    """Dense MPO tensor (l, s, r, s') from the text format of the reference's
+    cylinder fixtures (tests/2dHeisenberg/mpotensor_*.txt, read by
+    tests/2Dheisenberg.cpp:130-175): one non-zero per line, "l s s' r value",
+    1-based indices, optionally wrapped in a C++ raw-string literal.  Note
+    the order of the indices on a line: the third number is the output
+    physical index s' and the fourth the right bond.  Bond dimensions are the
+    largest indices that occur."""
+    body = text.strip()
+    if body.startswith('R"('):
+        body = body[3:]
+    if body.endswith(')"'):
+        body = body[:-2]
+    rows = [ln.split() for ln in body.splitlines() if len(ln.split()) >= 5]
+    idx = np.array([[int(x) for x in r[:4]] for r in rows], dtype=np.int64).reshape(-1, 4)
+    val = np.array([float(r[4]) for r in rows], dtype=np.float64)
+    if idx.size and idx.min() < 1:
+        raise ValueError("indices are 1-based")
+    lb, d, rb = (int(idx[:, 0].max()), max(2, int(idx[:, 1:3].max())), int(idx[:, 3].max())) if idx.size else (1, 2, 1)
+    W = np.zeros((lb, d, rb, d))
+    W[idx[:, 0] - 1, idx[:, 1] - 1, idx[:, 3] - 1, idx[:, 2] - 1] = val
+    return W
+
+
 def chain_bonds(L):
     return [(i, i + 1) for i in range(L - 1)]
 
