@@ -208,3 +208,21 @@ def test_svd_large_graded_sector(ctx, m, n, decades):
     Vd = V.to_dense()
     assert np.abs(Ud.T @ Ud - np.eye(k)).max() < 1e-10
     assert np.abs(Vd.T @ Vd - np.eye(k)).max() < 1e-12
+
+
+def test_eigh_wide_sector(ctx):
+    """eigh of a sector wider than one work-group's LDS (k = 180): values
+    against the dense eigvalsh within 1e-12 of the norm, A.U = U.Lambda."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(77)
+    n = 180
+    M = rng.standard_normal((n, n))
+    A = BTensor(G1, [[n], [n]], [[(0,)], [(0,)]], (0,))
+    A.blocks[(0, 0)] = M + M.T
+    d, U = (from_dev(x) for x in qa.eigh(to_dev(ctx, A), 1))
+    w = np.linalg.eigvalsh(A.blocks[(0, 0)])
+    nrm = np.linalg.norm(A.blocks[(0, 0)], 2)
+    assert np.abs(np.sort(d.blocks[(0,)]) - w).max() <= 1e-12 * nrm
+    Ud = U.to_dense()
+    assert np.abs(A.blocks[(0, 0)] @ Ud - Ud * d.blocks[(0,)]).max() <= 1e-11 * nrm
+    assert np.abs(Ud.T @ Ud - np.eye(n)).max() < 1e-12
