@@ -561,7 +561,202 @@ __host__ __device__ inline int gram_eig_gsz(int k, int threads) {
 }
 
 
-__global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__ base,
+// ---- wave-local block Jacobi (k >= kWlMinK) ---------------------------------
+// The columns of X form NB = 2*ceil(k/16) blocks of 8 (columns >= k are zero
+// and never stored).  A work-group round pairs the blocks by the circle
+// ordering, one block pair (P, Q) per wave.  Inside the wave, 8-lane group g
+// keeps column g of P and column g of Q in registers (rows li + 8j), so the 64
+// column pairs across P and Q take 8 steps with no barrier: between steps the
+// Q columns move between groups by a lane xor (group offsets in Gray-code
+// order 0,1,3,2,6,7,5,4, so every group meets every Q column once).  The first
+// round of a sweep also takes the 28 pairs inside each block, by xor pairing
+// of the groups (both groups of a pair compute the same rotation from the same
+// sums and update their own column).  One barrier per round instead of one per
+// column pair, and the columns cross LDS once per round.
+constexpr int kWlMinK = 48;
+// Nine waves: one per block pair at k = 144 (18 blocks), and at most three
+// waves per SIMD, so each keeps two 18-row columns and the shifted copy in
+// registers without spilling.
+constexpr int kGramEigThreads = 576;
+static_assert(kGramEigMaxK <= 8 * 18, "the widest wave-local variant holds 18 rows per lane");
+
+__device__ __forceinline__ double dpp_ror8(double v) {  // lane i <- lane i ^ 8 within 16-lane rows
+  const long long b = __double_as_longlong(v);
+  int lo = int(b & 0xffffffffLL), hi = int(b >> 32);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x128, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x128, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+template <int M>
+__device__ __forceinline__ double xor_groups(double v) {  // lane i <- lane i ^ (8 M)
+  if constexpr (M == 1)
+    return dpp_ror8(v);
+  else
+    return __shfl_xor(v, 8 * M);
+}
+
+__device__ __forceinline__ void sum8(double& a, double& b, double& c) {
+#pragma unroll
+  for (int st = 0; st < 3; ++st) {
+    const double ta = dpp_bits(a, st), tb = dpp_bits(b, st), tc = dpp_bits(c, st);
+    a += ta;
+    b += tb;
+    c += tc;
+  }
+}
+
+// Column c of X (pitch k) into rows li + 8j.  Rows past k read the next
+// column (or, for the last one, the arrays that follow X in LDS) and are
+// masked; columns past k are zero.  One base address per column keeps the
+// loads on immediate offsets.
+template <int R>
+__device__ __forceinline__ void wl_load(const double* X, int k, int c, int li, double (&v)[R]) {
+  const bool cin = c < k;
+  const double* col = X + (cin ? c * k : 0) + li;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const double x = col[8 * j];
+    v[j] = (cin && li + 8 * j < k) ? x : 0.0;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void wl_store(double* X, int k, int c, int li, const double (&v)[R]) {
+  if (c >= k) return;
+  double* col = X + c * k + li;
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    if (li + 8 * j < k) col[8 * j] = v[j];
+}
+
+// Rotation of the column pair (x, y) given their sums xx, yy, xy: returns
+// false when the pair is orthogonal to tol_rot.
+__device__ __forceinline__ bool wl_rotation(double xx, double yy, double xy, double tol_rot, double& c, double& sn) {
+  if (xx > 0.0 && yy > 0.0 && xy * xy > (tol_rot * xx) * (tol_rot * yy)) {
+    jacobi_cs(yy - xx, 2.0 * xy, c, sn);
+    return true;
+  }
+  return false;
+}
+
+// Pairs (g, g ^ M) inside one block: x is the lower group's column.
+template <int R, int M>
+__device__ __forceinline__ void wl_intra(double (&v)[R], int grp, double tol_rot, int& cnt) {
+  const bool lower = grp < (grp ^ M);
+  double p[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) p[j] = xor_groups<M>(v[j]);
+  double oo = 0.0, pp = 0.0, op = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    oo += v[j] * v[j];
+    pp += p[j] * p[j];
+    op += v[j] * p[j];
+  }
+  sum8(oo, pp, op);
+  double c, sn;
+  if (wl_rotation(lower ? oo : pp, lower ? pp : oo, op, tol_rot, c, sn)) {
+    const double s2 = lower ? -sn : sn;  // x' = c x - s y, y' = s x + c y
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = c * v[j] + s2 * p[j];
+    cnt += lower ? 1 : 0;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void wl_intra_all(double (&v)[R], int grp, double tol_rot, int& cnt) {
+  wl_intra<R, 1>(v, grp, tol_rot, cnt);
+  wl_intra<R, 2>(v, grp, tol_rot, cnt);
+  wl_intra<R, 3>(v, grp, tol_rot, cnt);
+  wl_intra<R, 4>(v, grp, tol_rot, cnt);
+  wl_intra<R, 5>(v, grp, tol_rot, cnt);
+  wl_intra<R, 6>(v, grp, tol_rot, cnt);
+  wl_intra<R, 7>(v, grp, tol_rot, cnt);
+}
+
+template <int R>
+__device__ __forceinline__ void wl_cross(double (&a)[R], double (&b)[R], double tol_rot, int& cnt) {
+  double aa = 0.0, bb = 0.0, ab = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    aa += a[j] * a[j];
+    bb += b[j] * b[j];
+    ab += a[j] * b[j];
+  }
+  sum8(aa, bb, ab);
+  double c, sn;
+  if (wl_rotation(aa, bb, ab, tol_rot, c, sn)) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const double x = a[j], y = b[j];
+      a[j] = c * x - sn * y;
+      b[j] = sn * x + c * y;
+    }
+    ++cnt;
+  }
+}
+
+template <int R, int M>
+__device__ __forceinline__ void wl_shift(double (&b)[R]) {
+#pragma unroll
+  for (int j = 0; j < R; ++j) b[j] = xor_groups<M>(b[j]);
+}
+
+// Sweeps until one rotates nothing (or max_sweeps); nrot[2] are shared
+// counters, zero on entry.  Every thread of the block calls this.
+template <int R>
+__device__ __forceinline__ void wl_sweeps(double* X, int k, double tol_rot, int max_sweeps, int* nrot) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, grp = lane >> 3, li = lane & 7;
+  const int NB = 2 * ((k + 15) / 16);
+  const bool active = wave < NB / 2;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    int cnt = 0;
+    for (int t = 0; t < NB - 1; ++t) {
+      if (active) {
+        const int P = circle_block(wave, t, NB), Q = circle_block(NB - 1 - wave, t, NB);
+        double a[R], b[R];
+        if (t == 0) {  // round 0 holds every block once: pairs inside the blocks
+          wl_load<R>(X, k, P * 8 + grp, li, a);
+          wl_intra_all<R>(a, grp, tol_rot, cnt);
+          wl_store<R>(X, k, P * 8 + grp, li, a);
+          wl_load<R>(X, k, Q * 8 + grp, li, b);
+          wl_intra_all<R>(b, grp, tol_rot, cnt);
+          wl_store<R>(X, k, Q * 8 + grp, li, b);
+        }
+        wl_load<R>(X, k, P * 8 + grp, li, a);  // the same lanes wrote these rows: in order
+        wl_load<R>(X, k, Q * 8 + grp, li, b);
+        // group g pairs its P column with the Q column of group g ^ gray(u)
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_shift<R, 1>(b);
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_shift<R, 2>(b);
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_shift<R, 1>(b);
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_shift<R, 4>(b);
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_shift<R, 1>(b);
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_shift<R, 2>(b);
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_shift<R, 1>(b);
+        wl_cross<R>(a, b, tol_rot, cnt);
+        wl_store<R>(X, k, P * 8 + grp, li, a);
+        wl_store<R>(X, k, Q * 8 + (grp ^ 4), li, b);  // gray(7) = 4
+      }
+      __syncthreads();
+    }
+    if (li == 0 && cnt) atomicAdd(&nrot[sweep & 1], cnt);
+    __syncthreads();
+    const int n = nrot[sweep & 1];
+    if (tid == 0) nrot[(sweep + 1) & 1] = 0;
+    __syncthreads();
+    if (n == 0) break;
+  }
+}
+
+__global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict__ base,
                                                             const SmallJacobi* __restrict__ js,
                                                             int* __restrict__ flags, double tol, int max_sweeps,
                                                             int precond) {
@@ -719,7 +914,15 @@ __global__ __launch_bounds__(kSmallThreads) void k_gram_eig(double* __restrict__
   // ---- 3. one-sided Jacobi on the columns of X ------------------------------
   const double tol_rot = tol * (1.0 / 16.0);
   const int pi = tid / gsz, li = tid % gsz;
-  for (int sweep = 0; sweep < max_sweeps && !ident; ++sweep) {
+  if (!ident && k >= kWlMinK) {
+    if (k <= 64)
+      wl_sweeps<8>(X, k, tol_rot, max_sweeps, nrot);
+    else if (k <= 96)
+      wl_sweeps<12>(X, k, tol_rot, max_sweeps, nrot);
+    else
+      wl_sweeps<18>(X, k, tol_rot, max_sweeps, nrot);
+  }
+  for (int sweep = 0; sweep < max_sweeps && !ident && k < kWlMinK; ++sweep) {
     int* cnt = &nrot[sweep & 1];
     for (int r = 0; r < ke - 1; ++r) {
       if (pi < npairs) {
@@ -870,7 +1073,7 @@ void gs_step(hipStream_t st, double* base, const void* descs, int n) {
 void gram_eig(hipStream_t st, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,
               int precond, size_t lds_bytes) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_gram_eig, dim3(n), dim3(kSmallThreads), lds_bytes, st, base,
+  hipLaunchKernelGGL(k_gram_eig, dim3(n), dim3(kGramEigThreads), lds_bytes, st, base,
                      static_cast<const SmallJacobi*>(descs), flags, tol, max_sweeps, precond);
 }
 

@@ -298,10 +298,11 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   const size_t pchol_limit = kern::pchol_lds_limit();
   std::vector<int> wide;
   std::vector<char> large(ns, 0);
+  const int eig_max_k = std::min(kern::kGramEigMaxK, env_int("QTT_SVD_EIG_MAX_K", kern::kGramEigMaxK));
   for (int i = 0; i < ns; ++i) {
     const auto& s = B.secs[i];
     if (eigh_mode || !use_small || s.k < 1) continue;
-    if (s.k <= kern::kGramEigMaxK && kern::gram_eig_lds_bytes(s.k) + lds_margin <= lds_limit) {
+    if (s.k <= eig_max_k && kern::gram_eig_lds_bytes(s.k) + lds_margin <= lds_limit) {
       wide.push_back(i);
     } else if (size_t(s.k) * 4 * sizeof(double) <= pchol_limit && env_int("QTT_SVD_LARGE_GRAM", 1)) {
       wide.push_back(i);

@@ -226,3 +226,36 @@ def test_eigh_wide_sector(ctx):
     Ud = U.to_dense()
     assert np.abs(A.blocks[(0, 0)] @ Ud - Ud * d.blocks[(0,)]).max() <= 1e-11 * nrm
     assert np.abs(Ud.T @ Ud - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("m,n,decades,zeros", [(300, 48, None, 0), (64, 200, 6, 0), (500, 65, 10, 0),
+                                               (97, 260, 3, 0), (400, 140, None, 0), (139, 150, 12, 0),
+                                               (300, 120, 4, 20), (90, 90, None, 0)])
+def test_svd_one_group_widths(ctx, m, n, decades, zeros):
+    """Sectors the one-work-group Gram eigen-solver takes (48 <= k <= 140,
+    every register width of its wave-local sweeps, odd k, k equal to the
+    widest): random (flat) and graded spectra, and a rank-deficient sector.
+    Singular values within 1e-12 of the largest against the dense SVD
+    (library routine), reconstruction and orthonormality within 1e-12."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(7 * m + n)
+    k = min(m, n)
+    if decades is None:
+        B = rng.standard_normal((m, n))
+    else:
+        U0, _ = np.linalg.qr(rng.standard_normal((m, k)))
+        V0, _ = np.linalg.qr(rng.standard_normal((n, k)))
+        s = 10.0 ** (-decades * np.arange(k) / k)
+        s[k - zeros:] = 0.0
+        B = (U0 * s) @ V0.T
+    A = BTensor(G1, [[m], [n]], [[(0,)], [(0,)]], (0,))
+    A.blocks[(0, 0)] = B
+    U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, A), 1))
+    dd = _check_svd(A, U, d, V, 1)
+    ref = np.linalg.svd(B, compute_uv=False)
+    got = np.sort(dd)[::-1]
+    assert got.size <= k and np.abs(got - ref[:got.size]).max() <= 1e-12 * ref[0]
+    assert np.all(ref[got.size:] <= 1e-12 * ref[0])
+    for t in (U, V):
+        M = t.to_dense()
+        assert np.abs(M.T @ M - np.eye(M.shape[1])).max() < (1e-10 if decades and decades > 8 else 1e-12)
