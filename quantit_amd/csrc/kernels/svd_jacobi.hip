@@ -588,12 +588,27 @@ __device__ __forceinline__ double dpp_ror8(double v) {  // lane i <- lane i ^ 8 
   return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
 
+// Lane permutations without index registers where the hardware allows:
+// xor 8 by a DPP row rotation, xor 16 / 24 by ds_swizzle (bit-mask mode,
+// and 0x1F, xor in bits 10-14), xor 32 by ds_bpermute.
+template <int PATTERN>
+__device__ __forceinline__ double swizzle64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_swizzle(int(b & 0xffffffffLL), PATTERN);
+  const int hi = __builtin_amdgcn_ds_swizzle(int(b >> 32), PATTERN);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+__device__ __forceinline__ double xor32(double v) { return __shfl_xor(v, 32); }
+
 template <int M>
 __device__ __forceinline__ double xor_groups(double v) {  // lane i <- lane i ^ (8 M)
-  if constexpr (M == 1)
-    return dpp_ror8(v);
-  else
-    return __shfl_xor(v, 8 * M);
+  if constexpr (M == 1) return dpp_ror8(v);
+  if constexpr (M == 2) return swizzle64<0x401F>(v);
+  if constexpr (M == 3) return swizzle64<0x601F>(v);
+  if constexpr (M == 4) return xor32(v);
+  if constexpr (M == 5) return xor32(dpp_ror8(v));
+  if constexpr (M == 6) return xor32(swizzle64<0x401F>(v));
+  return xor32(swizzle64<0x601F>(v));
 }
 
 __device__ __forceinline__ void sum8(double& a, double& b, double& c) {
@@ -762,7 +777,7 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
                                                             int precond) {
   extern __shared__ double lds[];
   __shared__ int nrot[2];
-  __shared__ int nviol, nabs, best, piv, rank;
+  __shared__ int nviol, nabs, best, piv;
   __shared__ double pmax;
   const SmallJacobi J = js[blockIdx.x];
   const int k = J.k;
@@ -793,7 +808,6 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
     nviol = 0;
     nabs = 0;
     nrot[0] = nrot[1] = 0;
-    rank = k;
   }
   __syncthreads();
   // ---- 1. orthogonality test on G -------------------------------------------
@@ -826,89 +840,112 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
   __syncthreads();
   const bool ident = nviol == 0 || k < 2;
   if (!ident && precond) {
-    // ---- 2. pivoted Cholesky, right-looking, in place ------------------------
-    // Rows and columns are swapped physically, so after step j column j holds
-    // L(j:k, j) below the diagonal and the trailing block is contiguous; perm
-    // maps positions to the original indices; vec holds the trailing diagonal.
-    for (int i = tid; i < k; i += blockDim.x) vec[i] = X[i * k + i];
-    __syncthreads();
-    for (int j = 0; j < k; ++j) {
-      if (tid < 64) {
-        // pivot: largest remaining diagonal, smallest position on ties.  The
-        // key keeps the top 54 bits of the value and the inverted position in
-        // the low 10 bits (k <= 1023); non-positive values never win over
-        // positive ones under a signed compare.
-        long long key = LLONG_MIN;
-        for (int i = j + tid; i < k; i += 64) {
-          const long long kk = (__double_as_longlong(vec[i]) & ~0x3FFLL) | (1023 - i);
-          key = kk > key ? kk : key;
-        }
-        for (int st = 0; st < 4; ++st) {
-          const long long o = __double_as_longlong(dpp_bits(__longlong_as_double(key), st));
-          key = o > key ? o : key;
-        }
-        for (int off = 16; off < 64; off <<= 1) {
-          const long long o = __shfl_xor(key, off);
-          key = o > key ? o : key;
-        }
-        if (tid == 0) {
-          const int pi_ = key == LLONG_MIN ? j : 1023 - int(key & 0x3FF);
-          piv = pi_;
-          pmax = vec[pi_];
-        }
+    // ---- 2. pivoted Cholesky, G held in registers ------------------------------
+    // Thread t owns an 8 x 5 tile of G.  Step j picks the largest remaining
+    // diagonal p (smallest index on ties), the owners of column p publish
+    // l = G(:, p) / sqrt(G(p, p)) (zero on rows already pivoted), every tile
+    // takes the rank-one update and vec keeps the remaining diagonal.  Column
+    // j of X receives l in the original row order, so X X^T = G and the
+    // Jacobi below needs no permutation; perm flags the pivoted rows here and
+    // is reset to the identity afterwards.
+    constexpr int TR = 8, TC = 5;
+    const int ntc = (k + TC - 1) / TC;
+    const bool owner = tid < ((k + TR - 1) / TR) * ntc;
+    const int tr = owner ? tid / ntc : 0, tc = owner ? tid % ntc : 0;
+    double g[TR][TC];
+#pragma unroll
+    for (int a = 0; a < TR; ++a)
+#pragma unroll
+      for (int b = 0; b < TC; ++b) {
+        const int i = tr * TR + a, l = tc * TC + b;
+        g[a][b] = (owner && i < k && l < k) ? X[l * k + i] : 0.0;
       }
-      __syncthreads();
+    for (int i = tid; i < k; i += blockDim.x) {
+      vec[i] = X[i * k + i];
+      perm[i] = 0;
+    }
+    __syncthreads();  // every read of G from X precedes the L columns written below
+    // Pivot of the next step, by wave 0: the largest vec entry, smallest index
+    // on ties.  The key keeps the top 54 bits of the value and the inverted
+    // index in the low 10 bits (k <= 1023); pivoted entries hold -1, and
+    // non-positive values never win over positive ones under a signed compare.
+    auto pick = [&]() {
+      long long key = LLONG_MIN;
+      for (int i = tid; i < k; i += 64) {
+        const long long kk = (__double_as_longlong(vec[i]) & ~0x3FFLL) | (1023 - i);
+        key = kk > key ? kk : key;
+      }
+      for (int st = 0; st < 4; ++st) {
+        const long long o = __double_as_longlong(dpp_bits(__longlong_as_double(key), st));
+        key = o > key ? o : key;
+      }
+      long long o = __double_as_longlong(swizzle64<0x401F>(__longlong_as_double(key)));
+      key = o > key ? o : key;
+      o = __double_as_longlong(xor32(__longlong_as_double(key)));
+      key = o > key ? o : key;
+      if (tid == 0) {
+        const int pi_ = key == LLONG_MIN ? -1 : 1023 - int(key & 0x3FF);
+        piv = pi_;
+        pmax = pi_ >= 0 ? vec[pi_] : 0.0;
+      }
+    };
+    if (tid < 64) pick();
+    __syncthreads();
+    int rk = k;
+    for (int j = 0; j < k; ++j) {
       if (!(pmax > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
-        if (tid == 0) rank = j;
+        rk = j;
         break;
       }
-      const int pv = piv;
-      if (pv != j) {  // symmetric swap of position j and pv (uniform branch)
-        for (int r = tid; r < k; r += blockDim.x) {
-          const double t = X[j * k + r];
-          X[j * k + r] = X[pv * k + r];
-          X[pv * k + r] = t;
+      const int p = piv;
+      const double sq = sqrt(pmax), isq = 1.0 / sq;
+      if (owner && tc == p / TC) {
+        // column p - tc TC of the tile by 0/1 weights: a select on the index
+        // would be folded into an indexed load and move the tile to scratch
+        double m[TC];
+#pragma unroll
+        for (int b = 0; b < TC; ++b) m[b] = (p - tc * TC == b) ? 1.0 : 0.0;
+#pragma unroll
+        for (int a = 0; a < TR; ++a) {
+          const int i = tr * TR + a;
+          if (i < k) {
+            double v = 0.0;
+#pragma unroll
+            for (int b = 0; b < TC; ++b) v += g[a][b] * m[b];
+            const double l = i == p ? sq : (perm[i] ? 0.0 : v * isq);
+            hh[i] = l;
+            X[j * k + i] = l;
+          }
         }
-        __syncthreads();
-        for (int c = tid; c < k; c += blockDim.x) {
-          const double t = X[c * k + j];
-          X[c * k + j] = X[c * k + pv];
-          X[c * k + pv] = t;
-        }
+      }
+      __syncthreads();
+      if (owner) {
+        double li[TR], lc[TC];
+#pragma unroll
+        for (int a = 0; a < TR; ++a) li[a] = tr * TR + a < k ? hh[tr * TR + a] : 0.0;
+#pragma unroll
+        for (int b = 0; b < TC; ++b) lc[b] = tc * TC + b < k ? hh[tc * TC + b] : 0.0;
+#pragma unroll
+        for (int a = 0; a < TR; ++a)
+#pragma unroll
+          for (int b = 0; b < TC; ++b) g[a][b] -= li[a] * lc[b];
+      }
+      if (tid < 64) {  // remaining diagonal, then the next pivot, in the same wave
+        for (int i = tid; i < k; i += 64)
+          if (i != p) vec[i] -= hh[i] * hh[i];
         if (tid == 0) {
-          const double t = vec[j];
-          vec[j] = vec[pv];
-          vec[pv] = t;
-          const int q = perm[j];
-          perm[j] = perm[pv];
-          perm[pv] = q;
+          vec[p] = -1.0;
+          perm[p] = 1;
         }
-        __syncthreads();
-      }
-      const double inv = 1.0 / pmax;
-      const int m = k - j - 1;
-      for (int e = tid; e < m * m; e += blockDim.x) {
-        const int ii = j + 1 + e % m, ll = j + 1 + e / m;
-        X[ll * k + ii] -= X[j * k + ii] * X[j * k + ll] * inv;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        pick();
       }
       __syncthreads();
-      const double ljj = sqrt(pmax);
-      for (int i = j + 1 + tid; i < k; i += blockDim.x) {
-        const double l = X[j * k + i] / ljj;
-        X[j * k + i] = l;
-        vec[i] -= l * l;
-      }
-      if (tid == 0) X[j * k + j] = ljj;
-      __syncthreads();
     }
-    __syncthreads();
-    {  // X = L: keep the lower triangle of the first rank columns
-      const int rk = rank;
-      for (int e = tid; e < k * k; e += blockDim.x) {
-        const int i = e % k, jc = e / k;
-        if (i < jc || jc >= rk) X[e] = 0.0;
-      }
-    }
+    // columns from the rank on are zero; perm back to the identity
+    for (int e = rk * k + tid; e < k * k; e += blockDim.x) X[e] = 0.0;
+    for (int i = tid; i < ke; i += blockDim.x) perm[i] = i;
     __syncthreads();
   }
   // ---- 3. one-sided Jacobi on the columns of X ------------------------------
