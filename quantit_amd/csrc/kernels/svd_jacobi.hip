@@ -611,14 +611,31 @@ __device__ __forceinline__ double xor_groups(double v) {  // lane i <- lane i ^ 
   return xor32(swizzle64<0x601F>(v));
 }
 
-__device__ __forceinline__ void sum8(double& a, double& b, double& c) {
+__device__ __forceinline__ void sum8(double& a) {
 #pragma unroll
-  for (int st = 0; st < 3; ++st) {
-    const double ta = dpp_bits(a, st), tb = dpp_bits(b, st), tc = dpp_bits(c, st);
-    a += ta;
-    b += tb;
-    c += tc;
-  }
+  for (int st = 0; st < 3; ++st) a += dpp_bits(a, st);
+}
+
+// A column in registers: true column = d * v (rows li + 8j), n = its squared
+// norm.  Rotations in the scaled form move two FMAs per row (x' = x - t r y
+// with r a ratio of scales) and fold the cosine into d; n follows the exact
+// identities |x'|^2 = |x|^2 - t x.y, |y'|^2 = |y|^2 + t x.y and is recomputed
+// from the rows when it falls below a quarter of its value (cancellation).
+// The scales start at 1 when a column is loaded, so with at most 15
+// rotations between a load and a store they stay above 0.7^15.
+template <int R>
+struct WlCol {
+  double v[R];
+  double d, n;
+};
+
+template <int R>
+__device__ __forceinline__ double wl_norm(const WlCol<R>& x) {
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) s += x.v[j] * x.v[j];
+  sum8(s);
+  return s * x.d * x.d;
 }
 
 // Column c of X (pitch k) into rows li + 8j.  Rows past k read the next
@@ -626,96 +643,118 @@ __device__ __forceinline__ void sum8(double& a, double& b, double& c) {
 // masked; columns past k are zero.  One base address per column keeps the
 // loads on immediate offsets.
 template <int R>
-__device__ __forceinline__ void wl_load(const double* X, int k, int c, int li, double (&v)[R]) {
+__device__ __forceinline__ void wl_load(const double* X, int k, int c, int li, WlCol<R>& x) {
   const bool cin = c < k;
   const double* col = X + (cin ? c * k : 0) + li;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    const double x = col[8 * j];
-    v[j] = (cin && li + 8 * j < k) ? x : 0.0;
+    const double v = col[8 * j];
+    x.v[j] = (cin && li + 8 * j < k) ? v : 0.0;
   }
+  x.d = 1.0;
+  x.n = wl_norm<R>(x);
 }
 
 template <int R>
-__device__ __forceinline__ void wl_store(double* X, int k, int c, int li, const double (&v)[R]) {
+__device__ __forceinline__ void wl_store(double* X, int k, int c, int li, const WlCol<R>& x) {
   if (c >= k) return;
   double* col = X + c * k + li;
 #pragma unroll
   for (int j = 0; j < R; ++j)
-    if (li + 8 * j < k) col[8 * j] = v[j];
+    if (li + 8 * j < k) col[8 * j] = x.v[j] * x.d;
 }
 
-// Rotation of the column pair (x, y) given their sums xx, yy, xy: returns
-// false when the pair is orthogonal to tol_rot.
-__device__ __forceinline__ bool wl_rotation(double xx, double yy, double xy, double tol_rot, double& c, double& sn) {
+// Rotation of the column pair (x, y) from their squared norms and product:
+// t = tan(angle), c = cos(angle) as in jacobi_cs (x' = c x - s y,
+// y' = s x + c y, s = c t); false when the pair is orthogonal to tol_rot.
+__device__ __forceinline__ bool wl_rotation(double xx, double yy, double xy, double tol_rot, double& t, double& c) {
   if (xx > 0.0 && yy > 0.0 && xy * xy > (tol_rot * xx) * (tol_rot * yy)) {
-    jacobi_cs(yy - xx, 2.0 * xy, c, sn);
+    const double d = yy - xx, h = 2.0 * xy;
+    const double q2 = d * d + h * h;
+    const double r = q2 * fast_rsqrt(q2);
+    t = (d >= 0.0 ? h : -h) * fast_rcp(fabs(d) + r);
+    c = fast_rsqrt(1.0 + t * t);
     return true;
   }
   return false;
 }
 
-// Pairs (g, g ^ M) inside one block: x is the lower group's column.
+// Squared norm after a rotation, recomputed from the rows on cancellation.
+template <int R>
+__device__ __forceinline__ double wl_next_norm(const WlCol<R>& x, double n_old, double n_new) {
+  if (n_new < 0.25 * n_old) n_new = wl_norm<R>(x);  // uniform in the group
+  return n_new;
+}
+
+// Pairs (g, g ^ M) inside one block: x is the lower group's column.  Both
+// groups form the same sums (the products commute and the reduction order is
+// the same), so they compute the same rotation and each updates its own
+// column.
 template <int R, int M>
-__device__ __forceinline__ void wl_intra(double (&v)[R], int grp, double tol_rot, int& cnt) {
+__device__ __forceinline__ void wl_intra(WlCol<R>& x, int grp, double tol_rot, int& cnt) {
   const bool lower = grp < (grp ^ M);
   double p[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) p[j] = xor_groups<M>(v[j]);
-  double oo = 0.0, pp = 0.0, op = 0.0;
+  for (int j = 0; j < R; ++j) p[j] = xor_groups<M>(x.v[j]);
+  const double pd = xor_groups<M>(x.d), pn = xor_groups<M>(x.n);
+  double r = 0.0;
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    oo += v[j] * v[j];
-    pp += p[j] * p[j];
-    op += v[j] * p[j];
-  }
-  sum8(oo, pp, op);
-  double c, sn;
-  if (wl_rotation(lower ? oo : pp, lower ? pp : oo, op, tol_rot, c, sn)) {
-    const double s2 = lower ? -sn : sn;  // x' = c x - s y, y' = s x + c y
+  for (int j = 0; j < R; ++j) r += x.v[j] * p[j];
+  sum8(r);
+  const double xy = (x.d * pd) * r;
+  double t, c;
+  if (wl_rotation(lower ? x.n : pn, lower ? pn : x.n, xy, tol_rot, t, c)) {
+    // lower: x' = x - t (d_y / d_x) y;  upper: y' = y + t (d_x / d_y) x
+    const double f = (lower ? -t : t) * pd * fast_rcp(x.d);
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = c * v[j] + s2 * p[j];
+    for (int j = 0; j < R; ++j) x.v[j] += f * p[j];
+    x.d *= c;
+    x.n = wl_next_norm<R>(x, x.n, x.n + (lower ? -t : t) * xy);
     cnt += lower ? 1 : 0;
   }
 }
 
 template <int R>
-__device__ __forceinline__ void wl_intra_all(double (&v)[R], int grp, double tol_rot, int& cnt) {
-  wl_intra<R, 1>(v, grp, tol_rot, cnt);
-  wl_intra<R, 2>(v, grp, tol_rot, cnt);
-  wl_intra<R, 3>(v, grp, tol_rot, cnt);
-  wl_intra<R, 4>(v, grp, tol_rot, cnt);
-  wl_intra<R, 5>(v, grp, tol_rot, cnt);
-  wl_intra<R, 6>(v, grp, tol_rot, cnt);
-  wl_intra<R, 7>(v, grp, tol_rot, cnt);
+__device__ __forceinline__ void wl_intra_all(WlCol<R>& x, int grp, double tol_rot, int& cnt) {
+  wl_intra<R, 1>(x, grp, tol_rot, cnt);
+  wl_intra<R, 2>(x, grp, tol_rot, cnt);
+  wl_intra<R, 3>(x, grp, tol_rot, cnt);
+  wl_intra<R, 4>(x, grp, tol_rot, cnt);
+  wl_intra<R, 5>(x, grp, tol_rot, cnt);
+  wl_intra<R, 6>(x, grp, tol_rot, cnt);
+  wl_intra<R, 7>(x, grp, tol_rot, cnt);
 }
 
 template <int R>
-__device__ __forceinline__ void wl_cross(double (&a)[R], double (&b)[R], double tol_rot, int& cnt) {
-  double aa = 0.0, bb = 0.0, ab = 0.0;
+__device__ __forceinline__ void wl_cross(WlCol<R>& a, WlCol<R>& b, double tol_rot, int& cnt) {
+  double r = 0.0;
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    aa += a[j] * a[j];
-    bb += b[j] * b[j];
-    ab += a[j] * b[j];
-  }
-  sum8(aa, bb, ab);
-  double c, sn;
-  if (wl_rotation(aa, bb, ab, tol_rot, c, sn)) {
+  for (int j = 0; j < R; ++j) r += a.v[j] * b.v[j];
+  sum8(r);
+  const double ab = (a.d * b.d) * r;
+  double t, c;
+  if (wl_rotation(a.n, b.n, ab, tol_rot, t, c)) {
+    const double fa = -t * b.d * fast_rcp(a.d), fb = t * a.d * fast_rcp(b.d);
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const double x = a[j], y = b[j];
-      a[j] = c * x - sn * y;
-      b[j] = sn * x + c * y;
+      const double x = a.v[j], y = b.v[j];
+      a.v[j] = x + fa * y;
+      b.v[j] = y + fb * x;
     }
+    a.d *= c;
+    b.d *= c;
+    a.n = wl_next_norm<R>(a, a.n, a.n - t * ab);
+    b.n = wl_next_norm<R>(b, b.n, b.n + t * ab);
     ++cnt;
   }
 }
 
 template <int R, int M>
-__device__ __forceinline__ void wl_shift(double (&b)[R]) {
+__device__ __forceinline__ void wl_shift(WlCol<R>& b) {
 #pragma unroll
-  for (int j = 0; j < R; ++j) b[j] = xor_groups<M>(b[j]);
+  for (int j = 0; j < R; ++j) b.v[j] = xor_groups<M>(b.v[j]);
+  b.d = xor_groups<M>(b.d);
+  b.n = xor_groups<M>(b.n);
 }
 
 // Sweeps until one rotates nothing (or max_sweeps); nrot[2] are shared
@@ -730,7 +769,7 @@ __device__ __forceinline__ void wl_sweeps(double* X, int k, double tol_rot, int 
     for (int t = 0; t < NB - 1; ++t) {
       if (active) {
         const int P = circle_block(wave, t, NB), Q = circle_block(NB - 1 - wave, t, NB);
-        double a[R], b[R];
+        WlCol<R> a, b;
         if (t == 0) {  // round 0 holds every block once: pairs inside the blocks
           wl_load<R>(X, k, P * 8 + grp, li, a);
           wl_intra_all<R>(a, grp, tol_rot, cnt);
@@ -891,6 +930,10 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
     };
     if (tid < 64) pick();
     __syncthreads();
+    // Each step: (1) the owners of column p copy it to hh; (2) thread i forms
+    // l_i, writes it to nrm and to column j of X and updates vec[i]; (3) the
+    // tiles take the rank-one update while wave 0 picks the next pivot.  The
+    // phases carry no chains of dependent LDS accesses.
     int rk = k;
     for (int j = 0; j < k; ++j) {
       if (!(pmax > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
@@ -898,7 +941,7 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
         break;
       }
       const int p = piv;
-      const double sq = sqrt(pmax), isq = 1.0 / sq;
+      const double isq = fast_rsqrt(pmax), sq = pmax * isq;
       if (owner && tc == p / TC) {
         // column p - tc TC of the tile by 0/1 weights: a select on the index
         // would be folded into an indexed load and move the tile to scratch
@@ -907,40 +950,37 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
         for (int b = 0; b < TC; ++b) m[b] = (p - tc * TC == b) ? 1.0 : 0.0;
 #pragma unroll
         for (int a = 0; a < TR; ++a) {
-          const int i = tr * TR + a;
-          if (i < k) {
-            double v = 0.0;
+          double v = 0.0;
 #pragma unroll
-            for (int b = 0; b < TC; ++b) v += g[a][b] * m[b];
-            const double l = i == p ? sq : (perm[i] ? 0.0 : v * isq);
-            hh[i] = l;
-            X[j * k + i] = l;
-          }
+          for (int b = 0; b < TC; ++b) v += g[a][b] * m[b];
+          if (tr * TR + a < k) hh[tr * TR + a] = v;
+        }
+      }
+      __syncthreads();
+      for (int i = tid; i < k; i += blockDim.x) {
+        const double l = i == p ? sq : (perm[i] ? 0.0 : hh[i] * isq);
+        nrm[i] = l;
+        X[j * k + i] = l;
+        if (i == p) {
+          vec[p] = -1.0;
+          perm[p] = 1;
+        } else {
+          vec[i] -= l * l;  // pivoted rows hold -1 and l = 0
         }
       }
       __syncthreads();
       if (owner) {
         double li[TR], lc[TC];
 #pragma unroll
-        for (int a = 0; a < TR; ++a) li[a] = tr * TR + a < k ? hh[tr * TR + a] : 0.0;
+        for (int a = 0; a < TR; ++a) li[a] = tr * TR + a < k ? nrm[tr * TR + a] : 0.0;
 #pragma unroll
-        for (int b = 0; b < TC; ++b) lc[b] = tc * TC + b < k ? hh[tc * TC + b] : 0.0;
+        for (int b = 0; b < TC; ++b) lc[b] = tc * TC + b < k ? nrm[tc * TC + b] : 0.0;
 #pragma unroll
         for (int a = 0; a < TR; ++a)
 #pragma unroll
           for (int b = 0; b < TC; ++b) g[a][b] -= li[a] * lc[b];
       }
-      if (tid < 64) {  // remaining diagonal, then the next pivot, in the same wave
-        for (int i = tid; i < k; i += 64)
-          if (i != p) vec[i] -= hh[i] * hh[i];
-        if (tid == 0) {
-          vec[p] = -1.0;
-          perm[p] = 1;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        pick();
-      }
+      if (tid < 64) pick();
       __syncthreads();
     }
     // columns from the rank on are zero; perm back to the identity
