@@ -162,31 +162,49 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   const int wi = wave >> 1, wj = wave & 1;
   v4d g = v4d{0.0, 0.0, 0.0, 0.0};
   if (RESIDENT) {
-    // all 32 columns at once: every load in flight together
+    // all 32 columns at once: lane group (tid & 7) of column tid >> 3 takes
+    // rows (tid & 7) + 8u, so eight lanes read 64 contiguous bytes and up to
+    // 72 loads per thread are in flight together (one pass for the resident sizes)
     double* Yl = dyn;  // Yl[c * pitch + r]
-    constexpr int BATCH = 16;  // loads in flight per thread
-    const int total = NB2 * ell;
-    for (int e0 = tid; e0 < total; e0 += BATCH * 256) {
-      double v[BATCH];
+    {
+      constexpr int CH = 72;
+      const int c = tid >> 3, part = tid & 7;
+      const double* src = Y + int64_t(cols[c]) * ld;
+      double* dst = Yl + c * pitch;
+      for (int o = 0; o < ell; o += 8 * CH) {
+        double v[CH];
 #pragma unroll
-      for (int u = 0; u < BATCH; ++u) {
-        const int e = e0 + u * 256;
-        v[u] = e < total ? Y[int64_t(cols[e / ell]) * ld + e % ell] : 0.0;
-      }
+        for (int u = 0; u < CH; ++u) {
+          const int r = o + part + 8 * u;
+          const double x = src[r < ell ? r : ell - 1];
+          v[u] = r < ell ? x : 0.0;
+        }
 #pragma unroll
-      for (int u = 0; u < BATCH; ++u) {
-        const int e = e0 + u * 256;
-        if (e < total) Yl[(e / ell) * pitch + e % ell] = v[u];
+        for (int u = 0; u < CH; ++u) {
+          const int r = o + part + 8 * u;
+          if (r < ell) dst[r] = v[u];
+        }
       }
     }
     __syncthreads();
+    // four independent accumulation chains hide the MFMA latency
     const int cl = lane & 15, kq = lane >> 4;
-    for (int kk = 0; kk < ell; kk += 4) {
-      const int kr = kk + kq;
-      const double a = kr < ell ? Yl[(wi * 16 + cl) * pitch + kr] : 0.0;
-      const double b = kr < ell ? Yl[(wj * 16 + cl) * pitch + kr] : 0.0;
-      g = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, g, 0, 0, 0);
+    const double* ya = Yl + (wi * 16 + cl) * pitch + kq;
+    const double* yb = Yl + (wj * 16 + cl) * pitch + kq;
+    v4d g1 = v4d{0.0, 0.0, 0.0, 0.0}, g2 = g1, g3 = g1;
+    int kk = 0;
+    for (; kk + 16 <= ell; kk += 16) {
+      g = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk], yb[kk], g, 0, 0, 0);
+      g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk + 4], yb[kk + 4], g1, 0, 0, 0);
+      g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk + 8], yb[kk + 8], g2, 0, 0, 0);
+      g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk + 12], yb[kk + 12], g3, 0, 0, 0);
     }
+    for (; kk < ell; kk += 4) {
+      const bool in = kk + kq < ell;
+      const double xa = ya[in ? kk : 0], xb = yb[in ? kk : 0];
+      g = __builtin_amdgcn_mfma_f64_16x16x4f64(in ? xa : 0.0, in ? xb : 0.0, g, 0, 0, 0);
+    }
+    g = (g + g1) + (g2 + g3);
   } else {
     // YC-row chunks staged through LDS; the next chunk's global loads are
     // issued before the MFMAs of the current one
@@ -223,7 +241,7 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
     const int i = e / NB2, j = e % NB2;
     if (i < j) {
       const double gii = G[0][i][i], gjj = G[0][j][j], gij = G[0][i][j];
-      if (gii > 0.0 && gjj > 0.0 && fabs(gij) > tol * sqrt(gii) * sqrt(gjj)) flag = 1;
+      if (gii > 0.0 && gjj > 0.0 && gij * gij > (tol * gii) * (tol * gjj)) flag = 1;
     }
   }
   __syncthreads();
@@ -242,30 +260,36 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   }
   __syncthreads();
   {
+    // Step r: the 16 threads tid < 16 compute the rotations of the 16 pairs
+    // (one each) into rc/rs; after a barrier thread (P, R) applies J_P^T
+    // (rows) and J_R (columns) to its 2x2 block of G and J_R to the same rows
+    // of Q, in place.  The 256 blocks tile G and Q, so a step is two barriers
+    // and no chain of dependent LDS accesses beyond the diagonal reads.
+    __shared__ double rc[NB2 / 2], rs[NB2 / 2];
     const int P = tid >> 4, R = tid & 15;
-    auto rot = [&](int x, int y, double& c, double& sn) {
-      const double app = G[0][x][x], aqq = G[0][y][y], apq = G[0][x][y];
-      c = 1.0;
-      sn = 0.0;
-      if (app > 0.0 && aqq > 0.0 && apq * apq > (DBL_EPSILON * app) * (DBL_EPSILON * aqq)) {
-        jacobi_cs(aqq - app, 2.0 * apq, c, sn);
-        return true;
-      }
-      return false;
-    };
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
       if (tid == 0) nrot = 0;
       __syncthreads();
       for (int r = 0; r < NB2 - 1; ++r) {
+        if (tid < NB2 / 2) {
+          const int x0 = circle_block(tid, r, NB2), x1 = circle_block(NB2 - 1 - tid, r, NB2);
+          const int x = min(x0, x1), y = max(x0, x1);
+          const double app = G[0][x][x], aqq = G[0][y][y], apq = G[0][x][y];
+          double c = 1.0, sn = 0.0;
+          if (app > 0.0 && aqq > 0.0 && apq * apq > (DBL_EPSILON * app) * (DBL_EPSILON * aqq)) {
+            jacobi_cs(aqq - app, 2.0 * apq, c, sn);
+            atomicAdd(&nrot, 1);
+          }
+          rc[tid] = c;
+          rs[tid] = sn;
+        }
         const int p0 = circle_block(P, r, NB2), p1 = circle_block(NB2 - 1 - P, r, NB2);
         const int q0 = circle_block(R, r, NB2), q1 = circle_block(NB2 - 1 - R, r, NB2);
         const int a = min(p0, p1), a2 = max(p0, p1), b = min(q0, q1), b2 = max(q0, q1);
-        double cp, sp, cq, sq;
-        const bool rp = rot(a, a2, cp, sp);
-        rot(b, b2, cq, sq);
         const double g00 = G[0][a][b], g01 = G[0][a][b2], g10 = G[0][a2][b], g11 = G[0][a2][b2];
         const double q00 = Q[0][a][b], q01 = Q[0][a][b2], q10 = Q[0][a2][b], q11 = Q[0][a2][b2];
-        __syncthreads();  // every read of this round (diagonal blocks included) precedes the writes
+        __syncthreads();  // rotations published; every read of this step precedes the writes
+        const double cp = rc[P], sp = rs[P], cq = rc[R], sq = rs[R];
         // rows a, a2 by J_P^T; columns b, b2 by J_R (row' a = c a - s a2, row' a2 = c a2 + s a)
         const double h00 = cp * g00 - sp * g10, h01 = cp * g01 - sp * g11;
         const double h10 = cp * g10 + sp * g00, h11 = cp * g11 + sp * g01;
@@ -277,7 +301,6 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
         Q[0][a][b2] = cq * q01 + sq * q00;
         Q[0][a2][b] = cq * q10 - sq * q11;
         Q[0][a2][b2] = cq * q11 + sq * q10;
-        if (R == 0 && rp) atomicAdd(&nrot, 1);
         __syncthreads();
       }
       if (nrot == 0) break;
