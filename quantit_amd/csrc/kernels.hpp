@@ -171,6 +171,7 @@ struct CholPanel {
 };
 // One panel of nb columns starting at j0; LDS kmax * nb doubles.
 void pchol_panel(hipStream_t s, double* base, const void* descs, int n, int j0, int nb, int kmax);
+constexpr int kPcholExtra = 2;  // panel LDS per index besides the columns: running diagonal, flag
 size_t pchol_lds_limit();
 struct GramVec {
   int64_t x_off, w_off, scratch_off;  // scratch: 3k doubles

@@ -50,6 +50,13 @@ __device__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// 1/sqrt(x) by the hardware estimate and two Newton steps.
+__device__ __forceinline__ double fast_rsqrt_pc(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * (1.5 - 0.5 * x * y * y);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+
 }  // namespace
 
 constexpr int kLargeThreads = 1024;
@@ -103,84 +110,115 @@ void gram_check(hipStream_t st, const double* base, const void* descs, int n, in
 // state[1 + i] = 1 once original index i has been pivoted.  L is k x ldl
 // (column-major, leading dimension k), rows in the original order.  The
 // panel's columns are also written to Lp (k x nb, column-major) for the
-// trailing update.  LDS: nb columns of the panel, k doubles.
+// trailing update.  LDS: the nb panel columns, the running diagonal dg
+// (G_ii less this panel's columns so far) and the pivoted flags, k each.
+// A step: per-thread best key over its indices, wave maxima by DPP and lane
+// swizzles, one barrier, every thread reduces the wave maxima itself.  The
+// key is the value's top 51 bits with the inverted index in the low 13
+// (k <= 8192): the largest diagonal wins, the smallest index among values
+// equal to 2^-39 relative; non-positive values never beat positive ones.
+__device__ __forceinline__ long long pc_dpp(long long key, int ctrl) {
+  int lo = int(key & 0xffffffffLL), hi = int(key >> 32);
+  switch (ctrl) {
+    case 0:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, false);
+      break;
+    case 1:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, false);
+      break;
+    case 2:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0x141, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0x141, 0xF, 0xF, false);
+      break;
+    case 3:
+      lo = __builtin_amdgcn_mov_dpp(lo, 0x140, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_mov_dpp(hi, 0x140, 0xF, 0xF, false);
+      break;
+    case 4:  // lane xor 16 (ds_swizzle bit-mask mode)
+      lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
+      hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+      break;
+    default:  // lane xor 32
+      lo = __shfl_xor(lo, 32);
+      hi = __shfl_xor(hi, 32);
+      break;
+  }
+  return (long long)(unsigned)lo | ((long long)hi << 32);
+}
+
 __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restrict__ base,
                                                                const CholPanel* __restrict__ ps, int j0, int nb) {
   extern __shared__ double lds[];
-  __shared__ double sh[kLargeThreads];
-  __shared__ int sh_i[kLargeThreads];
+  __shared__ long long wkey[kLargeThreads / 64];
   const CholPanel P = ps[blockIdx.x];
   const int k = P.k;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
   const double* G = base + P.g_off;
   double* L = base + P.l_off;
   double* Lp = base + P.lp_off;
   int* state = reinterpret_cast<int*>(base + P.state_off);
-  double* col = lds;  // nb columns of length k
+  double* col = lds;       // nb columns of length k
+  double* dg = col + int64_t(nb) * k;
+  int* used = reinterpret_cast<int*>(dg + k);
   if (state[0] < k || j0 >= k) {  // already rank-deficient or finished: zero panel
     for (int64_t e = tid; e < int64_t(k) * nb; e += blockDim.x) Lp[e] = 0.0;
     return;
   }
+  for (int i = tid; i < k; i += blockDim.x) {
+    dg[i] = G[int64_t(i) * k + i];
+    used[i] = state[1 + i];
+  }
+  __syncthreads();
+  int rank = k;
   for (int jj = 0; jj < nb; ++jj) {
     const int j = j0 + jj;
     if (j >= k) {
       for (int i = tid; i < k; i += blockDim.x) col[jj * k + i] = 0.0;
       continue;
     }
-    // pivot: largest current diagonal among unused indices (G already holds
-    // the previous panels' updates; this panel's columns are subtracted here)
-    double bv = -1.0;
-    int bi = INT_MAX;
+    long long key = LLONG_MIN;
     for (int i = tid; i < k; i += blockDim.x) {
-      if (state[1 + i]) continue;
-      double v = G[int64_t(i) * k + i];
-      for (int t = 0; t < jj; ++t) v -= col[t * k + i] * col[t * k + i];
-      if (v > bv || (v == bv && i < bi)) {
-        bv = v;
-        bi = i;
-      }
+      const long long kk = used[i] ? LLONG_MIN : ((__double_as_longlong(dg[i]) & ~0x1FFFLL) | (8191 - i));
+      key = kk > key ? kk : key;
     }
-    sh[tid] = bv;
-    sh_i[tid] = bi;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        const double ov = sh[tid + w];
-        const int oi = sh_i[tid + w];
-        if (ov > sh[tid] || (ov == sh[tid] && oi < sh_i[tid])) {
-          sh[tid] = ov;
-          sh_i[tid] = oi;
-        }
-      }
-      __syncthreads();
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+      const long long o = pc_dpp(key, st);
+      key = o > key ? o : key;
     }
-    const double piv_v = sh[0];
-    const int p = sh_i[0];
+    if (lane == 0) wkey[wave] = key;
     __syncthreads();
-    if (!(piv_v > 0.0) || p == INT_MAX) {  // uniform: the rest is zero or negative noise
+    key = LLONG_MIN;
+    for (int w = 0; w < nwave; ++w) key = wkey[w] > key ? wkey[w] : key;
+    const int p = key == LLONG_MIN ? -1 : 8191 - int(key & 0x1FFF);
+    const double pv = p >= 0 ? dg[p] : 0.0;
+    if (!(pv > 0.0)) {  // uniform: the rest is zero or negative noise
       for (int t = jj; t < nb; ++t)
         for (int i = tid; i < k; i += blockDim.x) col[t * k + i] = 0.0;
-      if (tid == 0) state[0] = j;
+      rank = j;
       __syncthreads();
       break;
     }
-    const double ljj = sqrt(piv_v);
+    const double isq = fast_rsqrt_pc(pv), ljj = pv * isq;
     for (int i = tid; i < k; i += blockDim.x) {
       double v = 0.0;
       if (i == p) {
         v = ljj;
-      } else if (!state[1 + i]) {
+        used[p] = 1;
+      } else if (!used[i]) {
         v = G[int64_t(p) * k + i];
         for (int t = 0; t < jj; ++t) v -= col[t * k + i] * col[t * k + p];
-        v /= ljj;
+        v *= isq;
+        dg[i] -= v * v;
       }
       col[jj * k + i] = v;
     }
     __syncthreads();
-    if (tid == 0) state[1 + p] = 1;
-    __syncthreads();
   }
-  __syncthreads();
+  for (int i = tid; i < k; i += blockDim.x) state[1 + i] = used[i];
+  if (tid == 0 && rank < k) state[0] = rank;
   for (int64_t e = tid; e < int64_t(k) * nb; e += blockDim.x) {
     const int t = int(e / k), i = int(e % k);
     Lp[e] = col[e];
@@ -190,7 +228,7 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
 
 void pchol_panel(hipStream_t st, double* base, const void* descs, int n, int j0, int nb, int kmax) {
   if (n <= 0) return;
-  const size_t lds = size_t(kmax) * nb * sizeof(double);
+  const size_t lds = size_t(kmax) * (nb + kPcholExtra) * sizeof(double);
   hipLaunchKernelGGL(k_pchol_panel, dim3(n), dim3(kLargeThreads), lds, st, base,
                      static_cast<const CholPanel*>(descs), j0, nb);
 }
@@ -201,7 +239,7 @@ size_t pchol_lds_limit() {
     int dev = 0, v = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    limit = size_t(v) - 12 * 1024;  // static arrays of the kernel
+    limit = size_t(v) - 1024;  // static arrays of the kernel
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pchol_panel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(limit));
   }

@@ -304,7 +304,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     if (eigh_mode || !use_small || s.k < 1) continue;
     if (s.k <= eig_max_k && kern::gram_eig_lds_bytes(s.k) + lds_margin <= lds_limit) {
       wide.push_back(i);
-    } else if (size_t(s.k) * 4 * sizeof(double) <= pchol_limit && env_int("QTT_SVD_LARGE_GRAM", 1)) {
+    } else if (size_t(s.k) * (4 + kern::kPcholExtra) * sizeof(double) <= pchol_limit &&
+               env_int("QTT_SVD_LARGE_GRAM", 1)) {
       wide.push_back(i);
       large[i] = 1;
     }
@@ -314,7 +315,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     int kmax_large = 0;
     for (int i : wide)
       if (large[i]) kmax_large = std::max(kmax_large, B.secs[i].k);
-    const int pnb = kmax_large ? int(std::min<size_t>(32, pchol_limit / (size_t(kmax_large) * sizeof(double)))) : 0;
+    const int pnb =
+        kmax_large
+            ? int(std::min<size_t>(32, pchol_limit / (size_t(kmax_large) * sizeof(double)) - kern::kPcholExtra))
+            : 0;
     int64_t elems = 0;
     std::vector<int64_t> go(nw), gwo(nw), y2o(nw), w2o(nw), lo(nw, -1), lpo(nw, -1), sco(nw, -1), sto(nw, -1);
     size_t lds_need = 0;
