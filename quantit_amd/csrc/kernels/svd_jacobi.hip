@@ -136,7 +136,7 @@ template <bool RESIDENT>
 __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, const SvdSector* __restrict__ secs,
                                                    const int32_t* __restrict__ work, int round,
                                                    int* __restrict__ counters, double tol, int inner_sweeps,
-                                                   int pitch) {
+                                                   int pitch, int cross_after0) {
   extern __shared__ double dyn[];
   __shared__ double G[1][NB2][GP];
   __shared__ double Q[1][NB2][GP];
@@ -265,15 +265,31 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
     // (rows) and J_R (columns) to its 2x2 block of G and J_R to the same rows
     // of Q, in place.  The 256 blocks tile G and Q, so a step is two barriers
     // and no chain of dependent LDS accesses beyond the diagonal reads.
+    // With cross_after0, rounds after the first of a sweep take only the 256
+    // pairs across the two blocks (pairs (i, 16 + (i + r) mod 16), 16
+    // steps): the first round of every sweep holds each block once and takes
+    // all 496 pairs, so a sweep still meets every column pair.
     __shared__ double rc[NB2 / 2], rs[NB2 / 2];
     const int P = tid >> 4, R = tid & 15;
+    const bool cross = cross_after0 && round > 0;
+    const int nsteps = cross ? NB2 / 2 : NB2 - 1;
+    auto pair_of = [&](int i, int r, int& x, int& y) {
+      if (cross) {
+        x = i;
+        y = NB2 / 2 + ((i + r) & (NB2 / 2 - 1));
+      } else {
+        const int x0 = circle_block(i, r, NB2), x1 = circle_block(NB2 - 1 - i, r, NB2);
+        x = min(x0, x1);
+        y = max(x0, x1);
+      }
+    };
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
       if (tid == 0) nrot = 0;
       __syncthreads();
-      for (int r = 0; r < NB2 - 1; ++r) {
+      for (int r = 0; r < nsteps; ++r) {
         if (tid < NB2 / 2) {
-          const int x0 = circle_block(tid, r, NB2), x1 = circle_block(NB2 - 1 - tid, r, NB2);
-          const int x = min(x0, x1), y = max(x0, x1);
+          int x, y;
+          pair_of(tid, r, x, y);
           const double app = G[0][x][x], aqq = G[0][y][y], apq = G[0][x][y];
           double c = 1.0, sn = 0.0;
           if (app > 0.0 && aqq > 0.0 && apq * apq > (DBL_EPSILON * app) * (DBL_EPSILON * aqq)) {
@@ -283,9 +299,9 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
           rc[tid] = c;
           rs[tid] = sn;
         }
-        const int p0 = circle_block(P, r, NB2), p1 = circle_block(NB2 - 1 - P, r, NB2);
-        const int q0 = circle_block(R, r, NB2), q1 = circle_block(NB2 - 1 - R, r, NB2);
-        const int a = min(p0, p1), a2 = max(p0, p1), b = min(q0, q1), b2 = max(q0, q1);
+        int a, a2, b, b2;
+        pair_of(P, r, a, a2);
+        pair_of(R, r, b, b2);
         const double g00 = G[0][a][b], g01 = G[0][a][b2], g10 = G[0][a2][b], g11 = G[0][a2][b2];
         const double q00 = Q[0][a][b], q01 = Q[0][a][b2], q10 = Q[0][a2][b], q11 = Q[0][a2][b2];
         __syncthreads();  // rotations published; every read of this step precedes the writes
@@ -366,17 +382,17 @@ int svd_round_resident_rows() {
 }
 
 void svd_round(hipStream_t st, double* base, const SvdSector* secs, const int32_t* work, int nwork, int round,
-               int* counters, double tol, int inner_sweeps, int max_ell) {
+               int* counters, double tol, int inner_sweeps, int max_ell, int cross_after0) {
   if (nwork <= 0) return;
   const size_t staging = size_t(NB2) * YP * sizeof(double);
   if (max_ell <= svd_round_resident_rows()) {
     const int pitch = max_ell + 1;
     const size_t lds = std::max(size_t(NB2) * pitch * sizeof(double), staging);
     hipLaunchKernelGGL(k_svd_round<true>, dim3(nwork), dim3(256), lds, st, base, secs, work, round, counters, tol,
-                       inner_sweeps, pitch);
+                       inner_sweeps, pitch, cross_after0);
   } else {
     hipLaunchKernelGGL(k_svd_round<false>, dim3(nwork), dim3(256), staging, st, base, secs, work, round, counters,
-                       tol, inner_sweeps, 0);
+                       tol, inner_sweeps, 0, cross_after0);
   }
 }
 

@@ -98,6 +98,11 @@ void launch_gemm(Context* ctx, double* ab, const std::vector<kern::GemmOut>& out
                  ctx->upload_vec(pairs), alpha, beta);
 }
 
+int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : dflt;
+}
+
 // Cyclic block Jacobi sweeps over the sectors listed in `which` until a sweep
 // rotates nothing (or max_sweeps).  first_rot[i] receives the number of
 // rotating work-groups of the first sweep of sector which[i].  Returns the
@@ -113,6 +118,7 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   for (int i : which) active[i] = 1;
   std::vector<int32_t> h_cnt(ns);
   if (first_rot) first_rot->assign(which.size(), 0);
+  const int cross = env_int("QTT_SVD_CROSS_INNER", 1);
   int sweeps = 0;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     std::vector<int32_t> work;
@@ -132,7 +138,7 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     auto* d_work = ctx->upload_vec(work);
     for (int r = 0; r < max_nb - 1; ++r) {
       KernelTimer kt(*ctx, "svd_round");
-      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner, max_ell);
+      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner, max_ell, cross);
     }
     QTT_HIP(hipMemcpyAsync(h_cnt.data(), d_cnt, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
                            ctx->stream()));
@@ -148,11 +154,6 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     if (!any) break;
   }
   return sweeps;
-}
-
-int env_int(const char* name, int dflt) {
-  const char* v = std::getenv(name);
-  return v ? std::atoi(v) : dflt;
 }
 
 // Gathers the sector matrices of a rank-2 tensor and runs the batched Jacobi
