@@ -193,11 +193,35 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
     const double* yb = Yl + (wj * 16 + cl) * pitch + kq;
     v4d g1 = v4d{0.0, 0.0, 0.0, 0.0}, g2 = g1, g3 = g1;
     int kk = 0;
-    for (; kk + 16 <= ell; kk += 16) {
-      g = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk], yb[kk], g, 0, 0, 0);
-      g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk + 4], yb[kk + 4], g1, 0, 0, 0);
-      g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk + 8], yb[kk + 8], g2, 0, 0, 0);
-      g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ya[kk + 12], yb[kk + 12], g3, 0, 0, 0);
+    if (ell >= 16) {  // operands of the next 16 rows are read while the MFMAs run
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = ya[4 * u];
+        b[u] = yb[4 * u];
+      }
+      for (; kk + 32 <= ell; kk += 16) {
+        double an[4], bn[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          an[u] = ya[kk + 16 + 4 * u];
+          bn[u] = yb[kk + 16 + 4 * u];
+        }
+        g = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], g, 0, 0, 0);
+        g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], g1, 0, 0, 0);
+        g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], g2, 0, 0, 0);
+        g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], g3, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = an[u];
+          b[u] = bn[u];
+        }
+      }
+      g = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], g, 0, 0, 0);
+      g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], g1, 0, 0, 0);
+      g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], g2, 0, 0, 0);
+      g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], g3, 0, 0, 0);
+      kk += 16;
     }
     for (; kk < ell; kk += 4) {
       const bool in = kk + kq < ell;
@@ -326,22 +350,24 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   const int cur = 0;
   // ---- 4. rotate the columns of Y and W ---------------------------------------
   if (RESIDENT) {
+    // (Y Q)^T = Q^T Y^T tile by tile: the MFMA output then holds 16
+    // consecutive rows of one column per 16 lanes, so the stores coalesce
     const double* Yl = dyn;
     for (int l0 = 0; l0 < ell; l0 += 4 * 32) {
       v4d acc[2][2] = {{v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}},
                        {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}}};
       const int cl = lane & 15;
+      const int r0 = l0 + wave * 32 + cl, r1 = r0 + 16;
 #pragma unroll
       for (int kk = 0; kk < NB2; kk += 4) {
         const int kr = kk + (lane >> 4);
-        const int r0 = l0 + wave * 32 + cl, r1 = r0 + 16;
         const double a0 = r0 < ell ? Yl[kr * pitch + r0] : 0.0;
         const double a1 = r1 < ell ? Yl[kr * pitch + r1] : 0.0;
         const double b0 = Q[cur][kr][cl], b1 = Q[cur][kr][16 + cl];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
       }
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt)
@@ -349,8 +375,8 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
         for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int row = l0 + wave * 32 + rt * 16 + 4 * r + (lane >> 4);  // f64 MFMA output layout
-            const int col = jt * 16 + (lane & 15);
+            const int row = (rt ? r1 : r0);
+            const int col = jt * 16 + 4 * r + (lane >> 4);  // f64 MFMA output layout, transposed tile
             if (row < ell) Y[int64_t(cols[col]) * ld + row] = acc[rt][jt][r];
           }
     }
