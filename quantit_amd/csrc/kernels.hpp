@@ -54,7 +54,10 @@ struct ChunkTask {
   int32_t desc;
   int32_t chunk;
 };
-constexpr int kChunk = 8192;
+// Elements per work-group task of the element-wise kernels: 8 per thread of
+// a 256-thread group, so a task is a few dependent memory round trips and a
+// launch spreads over many work-groups.
+constexpr int kChunk = 2048;
 void permute_copy(hipStream_t s, double* base, const PermDesc* descs, const ChunkTask* tasks, int ntasks);
 
 // 2-D strided copy dst[r*dst_rs + c*dst_cs] = alpha * src[r*src_rs + c*src_cs];

@@ -1257,12 +1257,18 @@ __global__ __launch_bounds__(256) void k_svd_colnorms(double* __restrict__ base,
   double* sig = base + s.w_off + int64_t(kp) * kp;
   const int lane = threadIdx.x & 63;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int l = lane; l < ell; l += 64) {
+  const double* col[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (c0 + c < k) {
-        const double x = Y[int64_t(c0 + c) * ell + l];
-        acc[c] += x * x;
+  for (int c = 0; c < 4; ++c) col[c] = Y + int64_t(min(c0 + c, k - 1)) * ell;  // columns past k: masked below
+  for (int l = lane; l < ell; l += 256) {  // 16 loads in flight per lane
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = l + 64 * u;
+      const int rr = r < ell ? r : ell - 1;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double x = col[c][rr];
+        acc[c] += (r < ell && c0 + c < k) ? x * x : 0.0;
       }
     }
   }
