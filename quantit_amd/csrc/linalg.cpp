@@ -394,7 +394,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       int64_t pelems = 0;
       for (size_t j = 0; j < qs.size(); ++j) {
         const auto& s = B.secs[wide[qs[j]]];
-        parts[j] = int(std::min<int64_t>(8, std::max<int64_t>(1, s.ell / 512)));
+        parts[j] = int(std::min<int64_t>(16, std::max<int64_t>(1, s.ell / env_int("QTT_GRAM_ROWS_PER_PART", 256))));
         if (parts[j] > 1) pelems += int64_t(parts[j]) * s.k * s.k;
       }
       BufPtr pbuf = pelems ? ctx->alloc(size_t(pelems) * sizeof(double)) : nullptr;
