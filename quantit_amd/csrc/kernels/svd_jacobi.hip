@@ -1447,13 +1447,18 @@ void fill_i32(hipStream_t st, int32_t* p, int n, int32_t v) {
 }
 
 // ---- global truncation ------------------------------------------------------
-__global__ __launch_bounds__(256) void k_truncate(double* __restrict__ base, const int64_t* __restrict__ s_offs,
-                                                  const int32_t* __restrict__ ks, int nsec, int64_t scratch_off,
-                                                  int64_t total, double tol, double pw, int64_t min_size,
-                                                  int64_t max_size, int32_t* __restrict__ kept,
-                                                  double* __restrict__ thr_out) {
+// The values of all sectors are gathered, sorted descending (bitonic, in LDS
+// when 2^ceil(log2 total) doubles fit, else in the scratch area) and the cut
+// is found by the reference's sequential tail accumulation.
+template <bool IN_LDS>
+__global__ __launch_bounds__(1024) void k_truncate(double* __restrict__ base, const int64_t* __restrict__ s_offs,
+                                                   const int32_t* __restrict__ ks, int nsec, int64_t scratch_off,
+                                                   int64_t total, double tol, double pw, int64_t min_size,
+                                                   int64_t max_size, int32_t* __restrict__ kept,
+                                                   double* __restrict__ thr_out) {
+  extern __shared__ double vl[];
   __shared__ double thr_sh;
-  double* v = base + scratch_off;
+  double* v = IN_LDS ? vl : base + scratch_off;
   int64_t n2 = 1;
   while (n2 < total) n2 <<= 1;
   const int tid = threadIdx.x;
@@ -1503,19 +1508,45 @@ __global__ __launch_bounds__(256) void k_truncate(double* __restrict__ base, con
   }
   __syncthreads();
   const double thr = thr_sh;
-  for (int i = tid; i < nsec; i += blockDim.x) {
+  // kept[i] = index of the first value of sector i that is not above thr
+  // (its length when none), one wave per sector, 64 values per ballot
+  const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  for (int i = wave; i < nsec; i += nw) {
     const double* src = base + s_offs[i];
-    int c = 0;
-    while (c < ks[i] && src[c] > thr) ++c;
-    kept[i] = c;
+    int c = ks[i];
+    for (int j0 = 0; j0 < ks[i]; j0 += 64) {
+      const int j = j0 + lane;
+      const unsigned long long m = __ballot(j < ks[i] && !(src[j < ks[i] ? j : 0] > thr));
+      if (m) {
+        c = j0 + __ffsll((long long)m) - 1;
+        break;
+      }
+    }
+    if (lane == 0) kept[i] = c;
   }
 }
 
 void truncate_select(hipStream_t st, double* base, const int64_t* s_offs, const int32_t* ks, int nsec,
                      int64_t scratch_off, int64_t total, double tol, double pw, int64_t min_size, int64_t max_size,
                      int32_t* kept, double* thr_out) {
-  hipLaunchKernelGGL(k_truncate, dim3(1), dim3(256), 0, st, base, s_offs, ks, nsec, scratch_off, total, tol, pw,
-                     min_size, max_size, kept, thr_out);
+  int64_t n2 = 1;
+  while (n2 < total) n2 <<= 1;
+  const size_t lds = size_t(n2) * sizeof(double);
+  static size_t limit = 0;
+  if (!limit) {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    limit = size_t(v) - 256;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_truncate<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit));
+  }
+  if (lds <= limit)
+    hipLaunchKernelGGL(k_truncate<true>, dim3(1), dim3(1024), lds, st, base, s_offs, ks, nsec, scratch_off, total,
+                       tol, pw, min_size, max_size, kept, thr_out);
+  else
+    hipLaunchKernelGGL(k_truncate<false>, dim3(1), dim3(1024), 0, st, base, s_offs, ks, nsec, scratch_off, total,
+                       tol, pw, min_size, max_size, kept, thr_out);
 }
 
 }  // namespace kern
