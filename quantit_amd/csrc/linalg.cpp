@@ -118,7 +118,9 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   for (int i : which) active[i] = 1;
   std::vector<int32_t> h_cnt(ns);
   if (first_rot) first_rot->assign(which.size(), 0);
-  const int cross = env_int("QTT_SVD_CROSS_INNER", 1);
+  // From the second sweep on, rounds after the first of a sweep rotate only
+  // the pairs across their two blocks (0: never, 1: from the first sweep).
+  const int cross = env_int("QTT_SVD_CROSS_INNER", 2);
   int sweeps = 0;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     std::vector<int32_t> work;
@@ -138,7 +140,8 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     auto* d_work = ctx->upload_vec(work);
     for (int r = 0; r < max_nb - 1; ++r) {
       KernelTimer kt(*ctx, "svd_round");
-      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner, max_ell, cross);
+      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner, max_ell,
+                      cross == 1 || (cross == 2 && sweep > 0));
     }
     QTT_HIP(hipMemcpyAsync(h_cnt.data(), d_cnt, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
                            ctx->stream()));
