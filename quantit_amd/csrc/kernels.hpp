@@ -161,12 +161,12 @@ void gram_eig(hipStream_t s, double* base, const void* descs, int n, int* flags,
 size_t gram_eig_lds_bytes(int k);
 // First-order Gram-Schmidt factor T (k x k at w_off) from the Gram matrix at
 // y_off (SmallJacobi with ell = k).
-void gs_step(hipStream_t s, double* base, const void* descs, int n);
+void gs_step(hipStream_t s, double* base, const void* descs, int n, int kmax);
 
 // ---- Gram eigenvectors for wide sectors (G in device memory) ---------------
 // Orthogonality test of G (SmallJacobi: y_off = G, k, ldy; flag_idx/aux_idx
 // as for gram_eig).
-void gram_check(hipStream_t s, const double* base, const void* descs, int n, int* flags, double tol);
+void gram_check(hipStream_t s, const double* base, const void* descs, int n, int* flags, double tol, int kmax);
 struct CholPanel {
   int64_t g_off;      // G (k x k, ld k), updated between panels
   int64_t l_off;      // L (k x k, ld k, zeroed), rows in the original order
@@ -182,7 +182,7 @@ struct GramVec {
   int64_t x_off, w_off, scratch_off;  // scratch: 3k doubles
   int32_t k, ldx, ldw, pad;
 };
-void gram_vectors(hipStream_t s, double* base, const void* descs, int n);  // largest dynamic LDS per work-group (bytes)
+void gram_vectors(hipStream_t s, double* base, const void* descs, int n, int kmax);  // largest dynamic LDS per work-group (bytes)
 
 // Global truncation (sources/btensor_linalg.cpp:657-755,
 // sources/LinearAlgebra.cpp:57-75): singular values of nsec sectors stored

@@ -1194,22 +1194,31 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
 __global__ __launch_bounds__(256) void k_gs_step(double* __restrict__ base, const SmallJacobi* __restrict__ js) {
   const SmallJacobi J = js[blockIdx.x];
   const int k = J.k;
+  const int j0 = blockIdx.y * 8;  // eight columns of T per work-group
+  if (j0 >= k) return;
   const double* G = base + J.y_off;
   double* T = base + J.w_off;
-  for (int64_t e = threadIdx.x; e < int64_t(k) * k; e += blockDim.x) {
-    const int i = int(e % k), j = int(e / k);
-    double v = i == j ? 1.0 : 0.0;
-    if (i != j) {
-      const double di = G[int64_t(i) * J.ldy + i], dj = G[int64_t(j) * J.ldy + j];
-      if (di > 0.0 && (di > dj || (di == dj && i < j))) v = -G[int64_t(j) * J.ldy + i] / di;
+  double dj[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) dj[c] = j0 + c < k ? G[int64_t(j0 + c) * J.ldy + j0 + c] : 0.0;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    const double di = G[int64_t(i) * J.ldy + i];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int j = j0 + c;
+      if (j < k) {
+        double v = i == j ? 1.0 : 0.0;
+        if (i != j && di > 0.0 && (di > dj[c] || (di == dj[c] && i < j))) v = -G[int64_t(j) * J.ldy + i] / di;
+        T[int64_t(j) * J.ldw + i] = v;
+      }
     }
-    T[int64_t(j) * J.ldw + i] = v;
   }
 }
 
-void gs_step(hipStream_t st, double* base, const void* descs, int n) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_gs_step, dim3(n), dim3(256), 0, st, base, static_cast<const SmallJacobi*>(descs));
+void gs_step(hipStream_t st, double* base, const void* descs, int n, int kmax) {
+  if (n <= 0 || kmax <= 0) return;
+  hipLaunchKernelGGL(k_gs_step, dim3(n, (kmax + 7) / 8), dim3(256), 0, st, base,
+                     static_cast<const SmallJacobi*>(descs));
 }
 
 void gram_eig(hipStream_t st, double* base, const void* descs, int n, int* flags, double tol, int max_sweeps,

@@ -16,6 +16,7 @@
 //                   largest residual (two projection passes).
 #include <cfloat>
 #include <climits>
+#include <stdexcept>
 
 #include "../kernels.hpp"
 
@@ -62,28 +63,37 @@ __device__ __forceinline__ double fast_rsqrt_pc(double x) {
 constexpr int kLargeThreads = 1024;
 
 // flags[flag_idx] = pairs failing both tests, flags[aux_idx] = pairs passing
-// only the absolute one (same tests as k_gram_eig).
+// only the absolute one (same tests as k_gram_eig, on squares).  The
+// diagonal sits in LDS (dynamic, k doubles); wave w scans the upper part of
+// columns w, w + 16, ... with coalesced reads.
 __global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __restrict__ base,
                                                               const SmallJacobi* __restrict__ js,
                                                               int* __restrict__ flags, double tol) {
+  extern __shared__ double dg[];
   __shared__ double sh[kLargeThreads];
   __shared__ int nv, na;
   const SmallJacobi J = js[blockIdx.x];
   const int k = J.k;
   const double* G = base + J.y_off;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
   if (tid == 0) nv = na = 0;
   double dm = 0.0;
-  for (int i = tid; i < k; i += blockDim.x) dm = fmax(dm, G[int64_t(i) * J.ldy + i]);
-  const double sgmax = sqrt(block_max(dm, sh));
+  for (int i = tid; i < k; i += blockDim.x) {
+    dg[i] = G[int64_t(i) * J.ldy + i];
+    dm = fmax(dm, dg[i]);
+  }
+  const double g2max = block_max(dm, sh);  // also orders the dg writes before the reads
+  const double t2 = tol * tol;
   int my = 0, mya = 0;
-  for (int64_t e = tid; e < int64_t(k) * k; e += blockDim.x) {
-    const int r = int(e % k), c = int(e / k);
-    if (r < c) {
-      const double gii = G[int64_t(r) * J.ldy + r], gjj = G[int64_t(c) * J.ldy + c];
-      const double g = fabs(G[int64_t(c) * J.ldy + r]);
-      if (gii > 0.0 && gjj > 0.0 && g > tol * sqrt(gii) * sqrt(gjj)) {
-        if (g > tol * sqrt(fmax(gii, gjj)) * sgmax)
+  for (int c = wave; c < k; c += nwave) {
+    const double gjj = dg[c];
+    const double* col = G + int64_t(c) * J.ldy;
+#pragma unroll 4
+    for (int r = lane; r < c; r += 64) {
+      const double gii = dg[r], g = col[r];
+      const double g2 = g * g;
+      if (gii > 0.0 && gjj > 0.0 && g2 > t2 * gii * gjj) {
+        if (g2 > t2 * fmax(gii, gjj) * g2max)
           ++my;
         else
           ++mya;
@@ -99,9 +109,20 @@ __global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __re
   }
 }
 
-void gram_check(hipStream_t st, const double* base, const void* descs, int n, int* flags, double tol) {
+void gram_check(hipStream_t st, const double* base, const void* descs, int n, int* flags, double tol, int kmax) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_gram_check, dim3(n), dim3(kLargeThreads), 0, st, base,
+  static size_t limit = 0;
+  if (!limit) {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
+    limit = size_t(v) - kLargeThreads * sizeof(double) - 64;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram_check),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit));
+  }
+  const size_t lds = size_t(kmax) * sizeof(double);
+  if (lds > limit) throw std::runtime_error("gram check: sector wider than the LDS holds");
+  hipLaunchKernelGGL(k_gram_check, dim3(n), dim3(kLargeThreads), lds, st, base,
                      static_cast<const SmallJacobi*>(descs), flags, tol);
 }
 
@@ -246,34 +267,73 @@ size_t pchol_lds_limit() {
   return limit;
 }
 
-// W = columns of X (k x k at x_off, leading dimension ldx) normalised; zero
-// columns completed.  W is written at w_off with leading dimension ldw;
-// scratch_off: 3k doubles.
+// W = columns of X (k x k at x_off, leading dimension ldx) normalised, by
+// k_gram_vec_norm over (sector, 64-column) work-groups (wave w takes four
+// columns, rows in flight four at a time); then k_gram_vectors completes the
+// zero columns from the unit vector with the largest residual (two
+// projection passes), one work-group per sector, and returns at once when
+// there are none.  W is written at w_off with leading dimension ldw;
+// scratch_off: 3k doubles (norms, two work vectors).
+__global__ __launch_bounds__(kLargeThreads) void k_gram_vec_norm(double* __restrict__ base,
+                                                                 const GramVec* __restrict__ js) {
+  const GramVec J = js[blockIdx.x];
+  const int k = J.k;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c0 = blockIdx.y * 64 + wave * 4;
+  if (c0 >= k) return;
+  const double* X = base + J.x_off;
+  double* W = base + J.w_off;
+  double* nrm = base + J.scratch_off;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const double* col[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) col[c] = X + int64_t(min(c0 + c, k - 1)) * J.ldx;
+  for (int r0 = lane; r0 < k; r0 += 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = r0 + 64 * u;
+      const int rr = r < k ? r : k - 1;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double x = col[c][rr];
+        acc[c] += r < k ? x * x : 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double a = acc[c];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    acc[c] = sqrt(a);
+    if (lane == 0 && c0 + c < k) nrm[c0 + c] = acc[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (c0 + c >= k) break;
+    const double v = acc[c];
+    double* w = W + int64_t(c0 + c) * J.ldw;
+    for (int r = lane; r < k; r += 64) w[r] = v > 1e-300 ? col[c][r] / v : 0.0;
+  }
+}
+
 __global__ __launch_bounds__(kLargeThreads) void k_gram_vectors(double* __restrict__ base,
                                                                 const GramVec* __restrict__ js) {
   __shared__ double sh[kLargeThreads];
-  __shared__ int best;
+  __shared__ int best, nzero;
   const GramVec J = js[blockIdx.x];
   const int k = J.k;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
-  const double* X = base + J.x_off;
+  const int tid = threadIdx.x;
   double* W = base + J.w_off;
   double* nrm = base + J.scratch_off;  // k
   double* vec = nrm + k;               // k
   double* hh = vec + k;                // k
-  for (int c = wave; c < k; c += nwave) {  // one wave per column
-    double a = 0.0;
-    for (int r = lane; r < k; r += 64) a += X[int64_t(c) * J.ldx + r] * X[int64_t(c) * J.ldx + r];
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
-    if (lane == 0) nrm[c] = sqrt(a);
-  }
+  if (tid == 0) nzero = 0;
   __syncthreads();
-  for (int64_t e = tid; e < int64_t(k) * k; e += blockDim.x) {
-    const int c = int(e / k), r = int(e % k);
-    const double v = nrm[c];
-    W[int64_t(c) * J.ldw + r] = v > 1e-300 ? X[int64_t(c) * J.ldx + r] / v : 0.0;
-  }
+  int z = 0;
+  for (int c = tid; c < k; c += blockDim.x) z += nrm[c] > 1e-300 ? 0 : 1;
+  if (z) atomicAdd(&nzero, z);
   __syncthreads();
+  if (nzero == 0) return;  // uniform
   for (int c = 0; c < k; ++c) {
     if (nrm[c] > 1e-300) continue;  // uniform
     for (int r = tid; r < k; r += blockDim.x) {
@@ -317,8 +377,10 @@ __global__ __launch_bounds__(kLargeThreads) void k_gram_vectors(double* __restri
   }
 }
 
-void gram_vectors(hipStream_t st, double* base, const void* descs, int n) {
-  if (n <= 0) return;
+void gram_vectors(hipStream_t st, double* base, const void* descs, int n, int kmax) {
+  if (n <= 0 || kmax <= 0) return;
+  hipLaunchKernelGGL(k_gram_vec_norm, dim3(n, (kmax + 63) / 64), dim3(kLargeThreads), 0, st, base,
+                     static_cast<const GramVec*>(descs));
   hipLaunchKernelGGL(k_gram_vectors, dim3(n), dim3(kLargeThreads), 0, st, base,
                      static_cast<const GramVec*>(descs));
 }

@@ -551,8 +551,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         vd.push_back({gb + lo[q], gb + gwo[q], gb + sco[q], s.k, s.k, s.k, 0});
       }
       jacobi_sweeps(ctx, ab, gsec, all, tol, inner, max_sweeps, nullptr);
+      int kv = 0;
+      for (auto& v : vd) kv = std::max(kv, v.k);
       KernelTimer kt(*ctx, "svd_vectors");
-      kern::gram_vectors(ctx->stream(), ab, ctx->upload_vec(vd), int(vd.size()));
+      kern::gram_vectors(ctx->stream(), ab, ctx->upload_vec(vd), int(vd.size()), kv);
     };
     const int max_it = env_int("QTT_SVD_GRAM_ITERS", 6);
     for (int it = 0; it < max_it && !act.empty(); ++it) {
@@ -577,13 +579,15 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       if (!act_l.empty()) {
         cd_launched = 1;
         std::vector<kern::SmallJacobi> cd;
+        int kc = 0;
         for (int q : act_l) {
           const auto& s = B.secs[wide[q]];
           cd.push_back({gb + go[q], -1, s.k, s.k, s.k, s.k, q, nw + q});
+          kc = std::max(kc, s.k);
         }
         {
           KernelTimer kt(*ctx, "svd_check");
-          kern::gram_check(ctx->stream(), ab, ctx->upload_vec(cd), int(cd.size()), d_flags, tol);
+          kern::gram_check(ctx->stream(), ab, ctx->upload_vec(cd), int(cd.size()), d_flags, tol, kc);
         }
         read_flags();
         retire(act_l);
@@ -607,13 +611,15 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     for (int pass = 0; pass < 2 && !corr.empty(); ++pass) {
       if (pass > 0) gram_of(corr);  // the first pass reuses the last Gram matrix of the loop
       std::vector<kern::SmallJacobi> gd;
+      int kg = 0;
       for (int q : corr) {
         const auto& s = B.secs[wide[q]];
         gd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, -1, -1});
+        kg = std::max(kg, s.k);
       }
       {
         KernelTimer kt(*ctx, "svd_gs");
-        kern::gs_step(ctx->stream(), ab, ctx->upload_vec(gd), int(gd.size()));
+        kern::gs_step(ctx->stream(), ab, ctx->upload_vec(gd), int(gd.size()), kg);
       }
       std::vector<kern::GemmOut> outs;
       std::vector<kern::GemmPair> prs;
