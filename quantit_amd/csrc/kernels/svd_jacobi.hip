@@ -33,20 +33,21 @@ constexpr int YC = 128;             // rows of Y staged per chunk
 constexpr int YP = YC + 2;          // LDS pitch of the staged Y chunk
 constexpr int GP = NB2 + 1;         // LDS pitch of G and Q
 
+// W = I (kp x kp) for every sector; gridDim.y work-groups share a sector,
+// one column per wave-step.
 __global__ void k_svd_init_w(double* __restrict__ base, const SvdSector* __restrict__ secs) {
   const SvdSector s = secs[blockIdx.x];
   const int kp = s.nb * kSvdBlock;
   double* W = base + s.w_off;
-  const int64_t n = int64_t(kp) * kp;
-  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const int64_t r = e % kp, c = e / kp;
-    W[e] = (r == c) ? 1.0 : 0.0;
-  }
+  const int wave = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nw = gridDim.y * (blockDim.x >> 6);
+  for (int c = wave; c < kp; c += nw)
+    for (int r = lane; r < kp; r += 64) W[int64_t(c) * kp + r] = (r == c) ? 1.0 : 0.0;
 }
 
 void svd_init_w(hipStream_t st, double* base, const SvdSector* secs, int nsec) {
   if (nsec <= 0) return;
-  hipLaunchKernelGGL(k_svd_init_w, dim3(nsec), dim3(256), 0, st, base, secs);
+  hipLaunchKernelGGL(k_svd_init_w, dim3(nsec, 16), dim3(256), 0, st, base, secs);
 }
 
 __device__ __forceinline__ int circle_block(int pos, int round, int nb) {
@@ -924,14 +925,18 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
     if ((tid & 63) == 0) atomicMax(reinterpret_cast<unsigned long long*>(nrm),
                                    static_cast<unsigned long long>(__double_as_longlong(dm)));
     __syncthreads();
-    const double sgmax = sqrt(nrm[0]);
+    // the two tests on squares: g^2 > tol^2 g_ii g_jj (relative) and
+    // g^2 > tol^2 max(g_ii, g_jj) max_l g_ll (absolute); wave w scans the
+    // upper part of columns w, w + nwave, ...
+    const double g2max = nrm[0], t2 = tol * tol;
+    const int lane = tid & 63, nwave = blockDim.x >> 6;
     int my = 0, mya = 0;
-    for (int e = tid; e < k * k; e += blockDim.x) {
-      const int r = e % k, c = e / k;
-      if (r < c) {
-        const double gii = X[r * k + r], gjj = X[c * k + c], g = fabs(X[e]);
-        if (gii > 0.0 && gjj > 0.0 && g > tol * sqrt(gii) * sqrt(gjj)) {
-          if (g > tol * sqrt(fmax(gii, gjj)) * sgmax)
+    for (int c = tid >> 6; c < k; c += nwave) {
+      const double gjj = X[c * k + c];
+      for (int r = lane; r < c; r += 64) {
+        const double gii = X[r * k + r], g = X[c * k + r], g2 = g * g;
+        if (gii > 0.0 && gjj > 0.0 && g2 > t2 * gii * gjj) {
+          if (g2 > t2 * fmax(gii, gjj) * g2max)
             ++my;
           else
             ++mya;
