@@ -305,8 +305,7 @@ BTensor clone(const BTensor& t) {
   }
   if (!tasks.empty()) {
     KernelTimer kt(*t.ctx, "copy2d");
-    auto* dd = t.ctx->upload_vec(cp);
-    auto* dt = t.ctx->upload_vec(tasks);
+    auto [dd, dt] = t.ctx->upload_pack(cp, tasks);
     kern::copy2d(t.ctx->stream(), ab, dd, dt, int(tasks.size()));
   }
   return out;
@@ -378,8 +377,7 @@ BTensor permute(const BTensor& t, const std::vector<int64_t>& in_perm) {
   }
   if (!tasks.empty()) {
     KernelTimer kt(*t.ctx, "permute_copy");
-    auto* dd = t.ctx->upload_vec(descs);
-    auto* dt = t.ctx->upload_vec(tasks);
+    auto [dd, dt] = t.ctx->upload_pack(descs, tasks);
     kern::permute_copy(t.ctx->stream(), ab, dd, dt, int(tasks.size()));
   }
   return out;

@@ -101,7 +101,8 @@ BTensor mul(const BTensor& x_in, const BTensor& y_in) {
   }
   if (!tasks.empty()) {
     KernelTimer kt(*ctx, "ew_mul");
-    kern::ew_binary(ctx->stream(), ab, ctx->upload_vec(descs), ctx->upload_vec(tasks), int(tasks.size()), 0, 0.0);
+    auto [up_descs, up_tasks] = ctx->upload_pack(descs, tasks);
+    kern::ew_binary(ctx->stream(), ab, up_descs, up_tasks, int(tasks.size()), 0, 0.0);
   }
   return out;
 }
@@ -136,7 +137,8 @@ BTensor div_scalar_tensor(const BTensor& x, const BTensor& s, double min_abs) {
   }
   if (!tasks.empty()) {
     KernelTimer kt(*ctx, "ew_div");
-    kern::ew_binary(ctx->stream(), ab, ctx->upload_vec(descs), ctx->upload_vec(tasks), int(tasks.size()), 1,
+    auto [up_descs, up_tasks] = ctx->upload_pack(descs, tasks);
+    kern::ew_binary(ctx->stream(), ab, up_descs, up_tasks, int(tasks.size()), 1,
                     min_abs);
   }
   return out;
@@ -190,7 +192,8 @@ BTensor merge_axpby(const BTensor& x, int64_t a_off, double a, const BTensor& y,
   }
   if (!tasks.empty()) {
     KernelTimer kt(*ctx, "axpby");
-    kern::axpby(ctx->stream(), ab, ctx->upload_vec(descs), ctx->upload_vec(tasks), int(tasks.size()), a_off, a,
+    auto [up_descs, up_tasks] = ctx->upload_pack(descs, tasks);
+    kern::axpby(ctx->stream(), ab, up_descs, up_tasks, int(tasks.size()), a_off, a,
                 b_off, b);
   }
   return out;
@@ -293,7 +296,8 @@ BTensor fill_like(Context* ctx, const Shape& s, double value) {
     std::vector<kern::ChunkTask> tasks;
     add_chunks(tasks, 0, n);
     KernelTimer kt(*ctx, "fill");
-    kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
+    auto [up_cp, up_tasks] = ctx->upload_pack(cp, tasks);
+    kern::copy2d(ctx->stream(), ab, up_cp, up_tasks, int(tasks.size()));
   }
   return out;
 }

@@ -82,9 +82,17 @@ struct Batched {
   int sweeps = 0;
 };
 
-void launch_gemm(Context* ctx, double* ab, const std::vector<kern::GemmOut>& outs,
-                 const std::vector<kern::GemmPair>& pairs, const char* name, double alpha = 1.0,
-                 double beta = 0.0) {
+// A grouped GEMM whose descriptors are on the device (one staging copy), so
+// it can be launched several times.
+struct GemmLaunch {
+  const kern::GemmTile* tiles = nullptr;
+  const kern::GemmOut* outs = nullptr;
+  const kern::GemmPair* pairs = nullptr;
+  int ntiles = 0;
+};
+
+GemmLaunch prepare_gemm(Context* ctx, const std::vector<kern::GemmOut>& outs,
+                        const std::vector<kern::GemmPair>& pairs) {
   std::vector<kern::GemmTile> tiles;
   for (size_t o = 0; o < outs.size(); ++o) {
     const int tm = (outs[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
@@ -92,10 +100,24 @@ void launch_gemm(Context* ctx, double* ab, const std::vector<kern::GemmOut>& out
     for (int i = 0; i < tm; ++i)
       for (int j = 0; j < tn; ++j) tiles.push_back({int32_t(o), int16_t(i), int16_t(j)});
   }
-  if (tiles.empty()) return;
+  GemmLaunch g;
+  if (tiles.empty()) return g;
+  std::tie(g.tiles, g.outs, g.pairs) = ctx->upload_pack(tiles, outs, pairs);
+  g.ntiles = int(tiles.size());
+  return g;
+}
+
+void run_gemm(Context* ctx, double* ab, const GemmLaunch& g, const char* name, double alpha = 1.0,
+              double beta = 0.0) {
+  if (g.ntiles == 0) return;
   KernelTimer kt(*ctx, name);
-  kern::gemm_f64(ctx->stream(), ab, ctx->upload_vec(tiles), int(tiles.size()), ctx->upload_vec(outs),
-                 ctx->upload_vec(pairs), alpha, beta);
+  kern::gemm_f64(ctx->stream(), ab, g.tiles, g.ntiles, g.outs, g.pairs, alpha, beta);
+}
+
+void launch_gemm(Context* ctx, double* ab, const std::vector<kern::GemmOut>& outs,
+                 const std::vector<kern::GemmPair>& pairs, const char* name, double alpha = 1.0,
+                 double beta = 0.0) {
+  run_gemm(ctx, ab, prepare_gemm(ctx, outs, pairs), name, alpha, beta);
 }
 
 int env_int(const char* name, int dflt) {
@@ -236,7 +258,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   }
   if (!tasks.empty()) {
     KernelTimer kt(*ctx, "svd_zero");
-    kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
+    {
+      auto [dc, dt] = ctx->upload_pack(cp, tasks);
+      kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
+    }
   }
   cp.clear();
   tasks.clear();
@@ -270,7 +295,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   }
   if (!tasks.empty()) {
     KernelTimer kt(*ctx, "svd_gather");
-    kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
+    {
+      auto [dc, dt] = ctx->upload_pack(cp, tasks);
+      kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
+    }
   }
   if (ns == 0) return B;
   {
@@ -374,7 +402,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         wi.push_back(w);
       }
       KernelTimer kt(*ctx, "svd_init");
-      kern::copy2d(ctx->stream(), ab, ctx->upload_vec(z), ctx->upload_vec(zt), int(zt.size()));
+      {
+        auto [dz, dzt] = ctx->upload_pack(z, zt);
+        kern::copy2d(ctx->stream(), ab, dz, dzt, int(zt.size()));
+      }
       kern::svd_init_w(ctx->stream(), ab, ctx->upload_vec(wi), nw);
     }
     std::vector<int64_t> alt_y(nw), alt_w(nw);
@@ -433,7 +464,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       launch_gemm(ctx, ab, outs, prs, "svd_gram");
       if (!st.empty()) {
         KernelTimer kt(*ctx, "svd_gram_sum");
-        kern::sum_parts(ctx->stream(), ab, ctx->upload_vec(sums), ctx->upload_vec(st), int(st.size()));
+        {
+          auto [ds, dst] = ctx->upload_pack(sums, st);
+          kern::sum_parts(ctx->stream(), ab, ds, dst, int(st.size()));
+        }
       }
     };
     auto read_flags = [&] {
@@ -500,7 +534,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
           z.push_back(c);
         }
         KernelTimer kt(*ctx, "svd_init");
-        kern::copy2d(ctx->stream(), ab, ctx->upload_vec(z), ctx->upload_vec(zt), int(zt.size()));
+        {
+        auto [dz, dzt] = ctx->upload_pack(z, zt);
+        kern::copy2d(ctx->stream(), ab, dz, dzt, int(zt.size()));
+      }
         for (int q : qs) {
           int32_t* st = reinterpret_cast<int32_t*>(ab + gb + sto[q]);
           kern::fill_i32(ctx->stream(), st, B.secs[wide[q]].k + 1, 0);
@@ -509,32 +546,33 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       }
       int kact = 0;
       for (int q : qs) kact = std::max(kact, B.secs[wide[q]].k);
+      // Descriptors of every panel step, uploaded once: a sector whose
+      // factorisation has ended sees a zero panel and a null update.
+      std::vector<kern::CholPanel> pd;
+      std::vector<kern::GemmOut> outs;
+      std::vector<kern::GemmPair> prs;
+      for (int q : qs) {
+        const auto& s = B.secs[wide[q]];
+        pd.push_back({gb + go[q], gb + lo[q], gb + lpo[q], gb + sto[q], s.k, 0});
+        kern::GemmPair gp{};  // G -= Lp Lp^T
+        gp.a_off = gb + lpo[q];
+        gp.a_rs = 1;
+        gp.a_cs = s.k;
+        gp.b_off = gb + lpo[q];
+        gp.b_rs = s.k;
+        gp.b_cs = 1;
+        gp.K = pnb;
+        outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
+        prs.push_back(gp);
+      }
+      const auto* d_pd = ctx->upload_vec(pd);
+      const GemmLaunch upd = prepare_gemm(ctx, outs, prs);
       for (int j0 = 0; j0 < kact; j0 += pnb) {
-        std::vector<kern::CholPanel> pd;
-        std::vector<kern::GemmOut> outs;
-        std::vector<kern::GemmPair> prs;
-        int kp_max = 0;
-        for (int q : qs) {
-          const auto& s = B.secs[wide[q]];
-          if (s.k <= j0) continue;
-          kp_max = std::max(kp_max, s.k);
-          pd.push_back({gb + go[q], gb + lo[q], gb + lpo[q], gb + sto[q], s.k, 0});
-          kern::GemmPair gp{};  // G -= Lp Lp^T
-          gp.a_off = gb + lpo[q];
-          gp.a_rs = 1;
-          gp.a_cs = s.k;
-          gp.b_off = gb + lpo[q];
-          gp.b_rs = s.k;
-          gp.b_cs = 1;
-          gp.K = pnb;
-          outs.push_back({gb + go[q], s.k, s.k, int32_t(prs.size()), int32_t(prs.size() + 1), s.k, 0});
-          prs.push_back(gp);
-        }
         {
           KernelTimer kt(*ctx, "svd_pchol");
-          kern::pchol_panel(ctx->stream(), ab, ctx->upload_vec(pd), int(pd.size()), j0, pnb, kp_max);
+          kern::pchol_panel(ctx->stream(), ab, d_pd, int(pd.size()), j0, pnb, kact);
         }
-        if (j0 + pnb < kact) launch_gemm(ctx, ab, outs, prs, "svd_pchol_update", -1.0, 1.0);
+        if (j0 + pnb < kact) run_gemm(ctx, ab, upd, "svd_pchol_update", -1.0, 1.0);
       }
       std::vector<kern::SvdSector> gsec;
       std::vector<int> all;
@@ -871,7 +909,8 @@ SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol,
   const int64_t clamp = std::numeric_limits<int64_t>::max();
   {
     KernelTimer kt(*ctx, "truncate_select");
-    kern::truncate_select(ctx->stream(), ab, ctx->upload_vec(s_offs), ctx->upload_vec(ks), nd,
+    auto [up_s_offs, up_ks] = ctx->upload_pack(s_offs, ks);
+    kern::truncate_select(ctx->stream(), ab, up_s_offs, up_ks, nd,
                           scratch->dptr() - ab, total, tol, pw, int64_t(std::min<size_t>(min_size, size_t(clamp))),
                           int64_t(std::min<size_t>(max_size, size_t(clamp))), d_kept, d_thr);
   }
@@ -938,7 +977,10 @@ SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol,
     }
     if (!tasks.empty()) {
       KernelTimer kt(*ctx, "truncate_copy");
-      kern::copy2d(ctx->stream(), ab, ctx->upload_vec(cp), ctx->upload_vec(tasks), int(tasks.size()));
+      {
+      auto [dc, dt] = ctx->upload_pack(cp, tasks);
+      kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
+    }
     }
     return out;
   };

@@ -6,6 +6,9 @@
 #include <memory>
 #include <mutex>
 #include <unordered_map>
+#include <cstring>
+#include <tuple>
+#include <vector>
 
 #include "qtt_common.hpp"
 
@@ -75,6 +78,25 @@ class Context {
   template <class T>
   T* upload_vec(const std::vector<T>& v) {
     return static_cast<T*>(upload(v.data(), v.size() * sizeof(T)));
+  }
+  // Several descriptor lists in one staging copy (one host-to-device
+  // transfer instead of one per list); each list starts 16-byte aligned.
+  template <class... T>
+  std::tuple<T*...> upload_pack(const std::vector<T>&... v) {
+    constexpr size_t kA = 16;
+    const size_t sizes[] = {v.size() * sizeof(T)...};
+    size_t offs[sizeof...(T)], total = 0;
+    for (size_t i = 0; i < sizeof...(T); ++i) {
+      offs[i] = total;
+      total += (sizes[i] + kA - 1) / kA * kA;
+    }
+    std::vector<char> buf(total ? total : 1);
+    const void* src[] = {static_cast<const void*>(v.data())...};
+    for (size_t i = 0; i < sizeof...(T); ++i)
+      if (sizes[i]) std::memcpy(buf.data() + offs[i], src[i], sizes[i]);
+    char* d = static_cast<char*>(upload(buf.data(), total));
+    size_t i = 0;
+    return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
   }
   void sync();
 

@@ -278,8 +278,7 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     for (auto& v : vb)
       if (v.need_pack) v.off += soff;
     KernelTimer kt(*ctx, "pack_permute");
-    auto* dd = ctx->upload_vec(pdesc);
-    auto* dt = ctx->upload_vec(ptasks);
+    auto [dd, dt] = ctx->upload_pack(pdesc, ptasks);
     kern::permute_copy(ctx->stream(), ab, dd, dt, int(ptasks.size()));
   }
 
@@ -347,32 +346,38 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     stats->pairs += long(gpairs.size());
     stats->tiles += long(tiles.size());
   }
-  auto* d_outs = ctx->upload_vec(gouts);
-  auto* d_pairs = ctx->upload_vec(gpairs);
   if (ctx->per_block_launch) {
-    // reference-shaped baseline: one launch per block pair (mm, then addmm_)
+    // reference-shaped baseline: one launch per block pair (mm, then addmm_);
+    // the per-pair descriptors and tile lists go up in one staging copy
     std::vector<kern::GemmOut> one;
+    std::vector<kern::GemmTile> tl;
+    std::vector<std::pair<int, int>> launches;  // (first tile, tile count) per pair
     for (size_t o = 0; o < outs.size(); ++o) {
       const int tm = (gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
       const int tn = (gouts[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
-      std::vector<kern::GemmTile> tl;
-      for (int i = 0; i < tm; ++i)
-        for (int j = 0; j < tn; ++j) tl.push_back({0, int16_t(i), int16_t(j)});
-      if (tl.empty()) continue;
-      auto* d_tl = ctx->upload_vec(tl);
+      if (tm * tn == 0) continue;
       for (int p = gouts[o].pair_begin; p < gouts[o].pair_end; ++p) {
+        const int t0 = int(tl.size());
+        for (int i = 0; i < tm; ++i)
+          for (int j = 0; j < tn; ++j) tl.push_back({int32_t(one.size()), int16_t(i), int16_t(j)});
         kern::GemmOut g1 = gouts[o];
         g1.pair_begin = p;
         g1.pair_end = p + 1;
-        auto* d_g1 = ctx->upload(&g1, sizeof(g1));
-        KernelTimer kt(*ctx, "gemm_per_block");
-        kern::gemm_f64(ctx->stream(), ab, d_tl, int(tl.size()), static_cast<kern::GemmOut*>(d_g1), d_pairs, 1.0,
-                       p == gouts[o].pair_begin ? 0.0 : 1.0);
+        one.push_back(g1);
+        launches.push_back({t0, tm * tn});
       }
+    }
+    if (one.empty()) return out;
+    auto [d_one, d_tl, d_pairs] = ctx->upload_pack(one, tl, gpairs);
+    for (size_t l = 0; l < launches.size(); ++l) {
+      const bool first = l == 0 || one[l].c_off != one[l - 1].c_off;
+      KernelTimer kt(*ctx, "gemm_per_block");
+      kern::gemm_f64(ctx->stream(), ab, d_tl + launches[l].first, launches[l].second, d_one, d_pairs, 1.0,
+                     first ? 0.0 : 1.0);
     }
     return out;
   }
-  auto* d_tiles = ctx->upload_vec(tiles);
+  auto [d_tiles, d_outs, d_pairs] = ctx->upload_pack(tiles, gouts, gpairs);
   if (ctx->precision == Precision::SPLIT_FP32) {
     KernelTimer kt(*ctx, "gemm_split");
     kern::gemm_split(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
