@@ -258,10 +258,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   }
   if (!tasks.empty()) {
     KernelTimer kt(*ctx, "svd_zero");
-    {
-      auto [dc, dt] = ctx->upload_pack(cp, tasks);
-      kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
-    }
+    auto [dc, dt] = ctx->upload_pack(cp, tasks);
+    kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
   }
   cp.clear();
   tasks.clear();
@@ -295,10 +293,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   }
   if (!tasks.empty()) {
     KernelTimer kt(*ctx, "svd_gather");
-    {
-      auto [dc, dt] = ctx->upload_pack(cp, tasks);
-      kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
-    }
+    auto [dc, dt] = ctx->upload_pack(cp, tasks);
+    kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
   }
   if (ns == 0) return B;
   {
@@ -402,10 +398,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         wi.push_back(w);
       }
       KernelTimer kt(*ctx, "svd_init");
-      {
-        auto [dz, dzt] = ctx->upload_pack(z, zt);
-        kern::copy2d(ctx->stream(), ab, dz, dzt, int(zt.size()));
-      }
+      auto [dz, dzt] = ctx->upload_pack(z, zt);
+      kern::copy2d(ctx->stream(), ab, dz, dzt, int(zt.size()));
       kern::svd_init_w(ctx->stream(), ab, ctx->upload_vec(wi), nw);
     }
     std::vector<int64_t> alt_y(nw), alt_w(nw);
@@ -464,10 +458,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       launch_gemm(ctx, ab, outs, prs, "svd_gram");
       if (!st.empty()) {
         KernelTimer kt(*ctx, "svd_gram_sum");
-        {
-          auto [ds, dst] = ctx->upload_pack(sums, st);
-          kern::sum_parts(ctx->stream(), ab, ds, dst, int(st.size()));
-        }
+        auto [ds, dst] = ctx->upload_pack(sums, st);
+        kern::sum_parts(ctx->stream(), ab, ds, dst, int(st.size()));
       }
     };
     auto read_flags = [&] {
@@ -534,10 +526,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
           z.push_back(c);
         }
         KernelTimer kt(*ctx, "svd_init");
-        {
         auto [dz, dzt] = ctx->upload_pack(z, zt);
         kern::copy2d(ctx->stream(), ab, dz, dzt, int(zt.size()));
-      }
         for (int q : qs) {
           int32_t* st = reinterpret_cast<int32_t*>(ab + gb + sto[q]);
           kern::fill_i32(ctx->stream(), st, B.secs[wide[q]].k + 1, 0);
@@ -977,10 +967,8 @@ SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol,
     }
     if (!tasks.empty()) {
       KernelTimer kt(*ctx, "truncate_copy");
-      {
       auto [dc, dt] = ctx->upload_pack(cp, tasks);
       kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
-    }
     }
     return out;
   };
