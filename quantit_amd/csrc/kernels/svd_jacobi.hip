@@ -87,16 +87,19 @@ __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, con
                                double (*Ys)[YP], const double (*Q)[GP]) {
   constexpr int PER = YC / 8;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c = tid >> 3, lq = (tid & 7) * PER;
+  const bool act = tid < 256;  // the first four waves work, every thread meets the barriers
+  const int c = act ? tid >> 3 : 0, lq = (tid & 7) * PER;
   const double* src = M + int64_t(cols[c]) * ld;
   double buf[PER];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) buf[i] = (lq + i < rows) ? src[lq + i] : 0.0;
+  for (int i = 0; i < PER; ++i) buf[i] = (act && lq + i < rows) ? src[lq + i] : 0.0;
   for (int l0 = 0; l0 < rows; l0 += YC) {
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) Ys[c][lq + i] = buf[i];
+      for (int i = 0; i < PER; ++i) Ys[c][lq + i] = buf[i];
+    }
     __syncthreads();
-    if (l0 + YC < rows) {
+    if (act && l0 + YC < rows) {
 #pragma unroll
       for (int i = 0; i < PER; ++i) {
         const int l = l0 + YC + lq + i;
@@ -106,7 +109,7 @@ __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, con
     v4d acc[2][2] = {{v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}},
                      {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}}};
 #pragma unroll
-    for (int kk = 0; kk < NB2; kk += 4) {
+    for (int kk = 0; kk < NB2 && act; kk += 4) {
       const int kr = kk + (lane >> 4), cl = lane & 15;
       const double a0 = Ys[kr][wave * 32 + cl], a1 = Ys[kr][wave * 32 + 16 + cl];
       const double b0 = Q[kr][cl], b1 = Q[kr][16 + cl];
@@ -116,6 +119,7 @@ __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, con
       acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
     __syncthreads();  // every wave has read its Ys rows before they are overwritten
+    if (!act) continue;
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
@@ -134,7 +138,7 @@ __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, con
 // computed from there and written back once; otherwise 128-row chunks are
 // staged twice (Gram, then rotation).
 template <bool RESIDENT>
-__global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, const SvdSector* __restrict__ secs,
+__global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __restrict__ base, const SvdSector* __restrict__ secs,
                                                    const int32_t* __restrict__ work, int round,
                                                    int* __restrict__ counters, double tol, int inner_sweeps,
                                                    int pitch, int cross_after0) {
@@ -163,73 +167,64 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
   const int wi = wave >> 1, wj = wave & 1;
   v4d g = v4d{0.0, 0.0, 0.0, 0.0};
   if (RESIDENT) {
-    // all 32 columns at once: lane group (tid & 7) of column tid >> 3 takes
-    // rows (tid & 7) + 8u, so eight lanes read 64 contiguous bytes and up to
-    // 72 loads per thread are in flight together (one pass for the resident sizes)
+    // all 32 columns at once: 32 lanes per column take rows (tid & 31) + 32u,
+    // up to 36 loads per thread in flight (one pass for the resident sizes)
     double* Yl = dyn;  // Yl[c * pitch + r]
     {
-      constexpr int CH = 72;
-      const int c = tid >> 3, part = tid & 7;
+      constexpr int CH = 36;
+      const int c = tid >> 5, part = tid & 31;
       const double* src = Y + int64_t(cols[c]) * ld;
       double* dst = Yl + c * pitch;
-      for (int o = 0; o < ell; o += 8 * CH) {
+      for (int o = 0; o < ell; o += 32 * CH) {
         double v[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-          const int r = o + part + 8 * u;
+          const int r = o + part + 32 * u;
           const double x = src[r < ell ? r : ell - 1];
           v[u] = r < ell ? x : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-          const int r = o + part + 8 * u;
+          const int r = o + part + 32 * u;
           if (r < ell) dst[r] = v[u];
         }
       }
     }
     __syncthreads();
-    // four independent accumulation chains hide the MFMA latency
+    // sixteen waves: four per 16x16 tile, each on the 4-row slices 4 qtr +
+    // 16 t, two accumulation chains each; the quarters are summed into G in
+    // a fixed order below
     const int cl = lane & 15, kq = lane >> 4;
-    const double* ya = Yl + (wi * 16 + cl) * pitch + kq;
-    const double* yb = Yl + (wj * 16 + cl) * pitch + kq;
-    v4d g1 = v4d{0.0, 0.0, 0.0, 0.0}, g2 = g1, g3 = g1;
-    int kk = 0;
-    if (ell >= 16) {  // operands of the next 16 rows are read while the MFMAs run
-      double a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a[u] = ya[4 * u];
-        b[u] = yb[4 * u];
-      }
-      for (; kk + 32 <= ell; kk += 16) {
-        double an[4], bn[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          an[u] = ya[kk + 16 + 4 * u];
-          bn[u] = yb[kk + 16 + 4 * u];
-        }
-        g = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], g, 0, 0, 0);
-        g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], g1, 0, 0, 0);
-        g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], g2, 0, 0, 0);
-        g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], g3, 0, 0, 0);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          a[u] = an[u];
-          b[u] = bn[u];
-        }
-      }
-      g = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], g, 0, 0, 0);
-      g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b[1], g1, 0, 0, 0);
-      g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], b[2], g2, 0, 0, 0);
-      g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], b[3], g3, 0, 0, 0);
-      kk += 16;
+    const int tile = wave & 3, qtr = wave >> 2;
+    const int ti = tile >> 1, tj = tile & 1;
+    const double* ya = Yl + (ti * 16 + cl) * pitch + kq;
+    const double* yb = Yl + (tj * 16 + cl) * pitch + kq;
+    v4d g1 = v4d{0.0, 0.0, 0.0, 0.0};
+    int kk = 4 * qtr;
+    for (; kk + 16 < ell; kk += 32) {
+      const bool in1 = kk + 16 + kq < ell;
+      const double a0 = ya[kk], b0 = yb[kk];
+      const double a1 = ya[in1 ? kk + 16 : 0], b1 = yb[in1 ? kk + 16 : 0];
+      g = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, g, 0, 0, 0);
+      g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(in1 ? a1 : 0.0, in1 ? b1 : 0.0, g1, 0, 0, 0);
     }
-    for (; kk < ell; kk += 4) {
+    for (; kk < ell; kk += 16) {
       const bool in = kk + kq < ell;
       const double xa = ya[in ? kk : 0], xb = yb[in ? kk : 0];
       g = __builtin_amdgcn_mfma_f64_16x16x4f64(in ? xa : 0.0, in ? xb : 0.0, g, 0, 0, 0);
     }
-    g = (g + g1) + (g2 + g3);
+    g = g + g1;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      if (qtr == qq) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double& dst = G[0][ti * 16 + 4 * r + (lane >> 4)][tj * 16 + (lane & 15)];
+          dst = qq == 0 ? g[r] : dst + g[r];
+        }
+      }
+      __syncthreads();
+    }
   } else {
     // YC-row chunks staged through LDS; the next chunk's global loads are
     // issued before the MFMAs of the current one
@@ -258,8 +253,10 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
       __syncthreads();
     }
   }
+  if (!RESIDENT) {  // the resident path has summed its quarters into G already
 #pragma unroll
-  for (int r = 0; r < 4; ++r) G[0][wi * 16 + 4 * r + (lane >> 4)][wj * 16 + (lane & 15)] = g[r];
+    for (int r = 0; r < 4; ++r) G[0][wi * 16 + 4 * r + (lane >> 4)][wj * 16 + (lane & 15)] = g[r];
+  }
   __syncthreads();
   // ---- 2. convergence test ---------------------------------------------------
   for (int e = tid; e < NB2 * NB2; e += blockDim.x) {
@@ -295,7 +292,8 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
     // steps): the first round of every sweep holds each block once and takes
     // all 496 pairs, so a sweep still meets every column pair.
     __shared__ double rc[NB2 / 2], rs[NB2 / 2];
-    const int P = tid >> 4, R = tid & 15;
+    const bool blk = tid < NB2 * NB2 / 4;  // 256 threads own the 2x2 blocks; the rest meet the barriers
+    const int P = blk ? tid >> 4 : 0, R = tid & 15;
     const bool cross = cross_after0 && round > 0;
     const int nsteps = cross ? NB2 / 2 : NB2 - 1;
     auto pair_of = [&](int i, int r, int& x, int& y) {
@@ -330,18 +328,20 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
         const double g00 = G[0][a][b], g01 = G[0][a][b2], g10 = G[0][a2][b], g11 = G[0][a2][b2];
         const double q00 = Q[0][a][b], q01 = Q[0][a][b2], q10 = Q[0][a2][b], q11 = Q[0][a2][b2];
         __syncthreads();  // rotations published; every read of this step precedes the writes
-        const double cp = rc[P], sp = rs[P], cq = rc[R], sq = rs[R];
-        // rows a, a2 by J_P^T; columns b, b2 by J_R (row' a = c a - s a2, row' a2 = c a2 + s a)
-        const double h00 = cp * g00 - sp * g10, h01 = cp * g01 - sp * g11;
-        const double h10 = cp * g10 + sp * g00, h11 = cp * g11 + sp * g01;
-        G[0][a][b] = cq * h00 - sq * h01;
-        G[0][a][b2] = cq * h01 + sq * h00;
-        G[0][a2][b] = cq * h10 - sq * h11;
-        G[0][a2][b2] = cq * h11 + sq * h10;
-        Q[0][a][b] = cq * q00 - sq * q01;
-        Q[0][a][b2] = cq * q01 + sq * q00;
-        Q[0][a2][b] = cq * q10 - sq * q11;
-        Q[0][a2][b2] = cq * q11 + sq * q10;
+        if (blk) {
+          const double cp = rc[P], sp = rs[P], cq = rc[R], sq = rs[R];
+          // rows a, a2 by J_P^T; columns b, b2 by J_R (row' a = c a - s a2, row' a2 = c a2 + s a)
+          const double h00 = cp * g00 - sp * g10, h01 = cp * g01 - sp * g11;
+          const double h10 = cp * g10 + sp * g00, h11 = cp * g11 + sp * g01;
+          G[0][a][b] = cq * h00 - sq * h01;
+          G[0][a][b2] = cq * h01 + sq * h00;
+          G[0][a2][b] = cq * h10 - sq * h11;
+          G[0][a2][b2] = cq * h11 + sq * h10;
+          Q[0][a][b] = cq * q00 - sq * q01;
+          Q[0][a][b2] = cq * q01 + sq * q00;
+          Q[0][a2][b] = cq * q10 - sq * q11;
+          Q[0][a2][b2] = cq * q11 + sq * q10;
+        }
         __syncthreads();
       }
       if (nrot == 0) break;
@@ -354,7 +354,8 @@ __global__ __launch_bounds__(256) void k_svd_round(double* __restrict__ base, co
     // (Y Q)^T = Q^T Y^T tile by tile: the MFMA output then holds 16
     // consecutive rows of one column per 16 lanes, so the stores coalesce
     const double* Yl = dyn;
-    for (int l0 = 0; l0 < ell; l0 += 4 * 32) {
+    const int nw = blockDim.x >> 6;
+    for (int l0 = 0; l0 < ell; l0 += nw * 32) {
       v4d acc[2][2] = {{v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}},
                        {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}}};
       const int cl = lane & 15;
@@ -415,7 +416,7 @@ void svd_round(hipStream_t st, double* base, const SvdSector* secs, const int32_
   if (max_ell <= svd_round_resident_rows()) {
     const int pitch = max_ell + 1;
     const size_t lds = std::max(size_t(NB2) * pitch * sizeof(double), staging);
-    hipLaunchKernelGGL(k_svd_round<true>, dim3(nwork), dim3(256), lds, st, base, secs, work, round, counters, tol,
+    hipLaunchKernelGGL(k_svd_round<true>, dim3(nwork), dim3(1024), lds, st, base, secs, work, round, counters, tol,
                        inner_sweeps, pitch, cross_after0);
   } else {
     hipLaunchKernelGGL(k_svd_round<false>, dim3(nwork), dim3(256), staging, st, base, secs, work, round, counters,
