@@ -2,6 +2,7 @@
 // context (device, HIP stream, arena, descriptor staging) and timing hooks.
 #pragma once
 
+#include <chrono>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -144,6 +145,27 @@ struct KernelTimer {
       ctx.record(name, a, b);
     }
   }
+};
+
+// RAII helper: wall time of a host-side phase (sector matching, descriptor
+// building) when profiling is enabled, folded into kernel_ms under a name
+// starting with "host_".  stop() ends the phase early.
+struct HostTimer {
+  Context& ctx;
+  const char* name;
+  bool on;
+  std::chrono::steady_clock::time_point t0;
+  HostTimer(Context& c, const char* n) : ctx(c), name(n), on(c.profiling) {
+    if (on) t0 = std::chrono::steady_clock::now();
+  }
+  void stop() {
+    if (!on) return;
+    on = false;
+    auto& e = ctx.kernel_ms[name];
+    e.first += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e.second += 1;
+  }
+  ~HostTimer() { stop(); }
 };
 
 }  // namespace qtt

@@ -70,6 +70,7 @@ struct KeyEnt {
 BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
                   const std::vector<int64_t>& db_in, ContractStats* stats) {
   Context* ctx = a.ctx;
+  HostTimer ht_match(*ctx, "host_td_match");
   if (a.shape.g != b.shape.g) throw CheckError("the two tensors have different type of conserved quantities");
   if (da_in.size() != db_in.size()) throw CheckError("both dimension lists should have the same length.");
   const int ra = a.rank(), rb = b.rank();
@@ -153,9 +154,10 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   };
   struct OutBlk {
     BIdx idx;
-    std::vector<std::pair<int, int>> pairs;  // (a block, b block)
+    int pb, pe;  // range in all_pairs
   };
   std::vector<OutBlk> outs;
+  std::vector<std::pair<int, int>> all_pairs;  // (a block, b block), grouped by output block
   std::vector<Match> matches;
   size_t s1 = 0;
   while (s1 < t1.size()) {
@@ -187,8 +189,10 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
       OutBlk ob;
       for (int j = 0; j < na; ++j) ob.idx.i[j] = t1[s1].key.i[j];
       for (int j = 0; j < nb; ++j) ob.idx.i[na + j] = t2[matches[m].e2].key.i[j];
-      for (size_t q = m; q < me; ++q) ob.pairs.push_back({t1[matches[q].e1].src, t2[matches[q].e2].src});
-      outs.push_back(std::move(ob));
+      ob.pb = int(all_pairs.size());
+      for (size_t q = m; q < me; ++q) all_pairs.push_back({t1[matches[q].e1].src, t2[matches[q].e2].src});
+      ob.pe = int(all_pairs.size());
+      outs.push_back(ob);
       m = me;
     }
     s1 = e1;
@@ -197,8 +201,13 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   std::vector<BIdx> oidx;
   oidx.reserve(outs.size());
   for (auto& o : outs) oidx.push_back(o.idx);
-  out.allocate(oidx);
+  ht_match.stop();
+  {
+    HostTimer ht(*ctx, "host_td_alloc");
+    out.allocate(oidx);
+  }
   if (outs.empty()) return out;
+  HostTimer ht_plan(*ctx, "host_td_plan");
 
   // matrix views of the operand blocks; blocks whose permuted view is not a
   // plain strided matrix are packed (row-major) into a scratch buffer
@@ -210,11 +219,10 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   };
   std::vector<View> va(a.blocks.size()), vb(b.blocks.size());
   std::vector<char> used_a(a.blocks.size(), 0), used_b(b.blocks.size(), 0);
-  for (auto& o : outs)
-    for (auto& p : o.pairs) {
-      used_a[p.first] = 1;
-      used_b[p.second] = 1;
-    }
+  for (auto& p : all_pairs) {
+    used_a[p.first] = 1;
+    used_b[p.second] = 1;
+  }
   std::vector<int> ca(da.begin(), da.end()), cb(db.begin(), db.end());
   std::vector<kern::PermDesc> pdesc;
   std::vector<kern::ChunkTask> ptasks;
@@ -278,15 +286,18 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     for (auto& v : vb)
       if (v.need_pack) v.off += soff;
     KernelTimer kt(*ctx, "pack_permute");
+    HostTimer ht(*ctx, "host_td_upload");
     auto [dd, dt] = ctx->upload_pack(pdesc, ptasks);
+    ht.stop();
     kern::permute_copy(ctx->stream(), ab, dd, dt, int(ptasks.size()));
   }
 
   // grouped GEMM descriptors
   std::vector<kern::GemmOut> gouts(outs.size());
   std::vector<kern::GemmPair> gpairs;
-  std::vector<kern::GemmTile> tiles;
-  std::vector<int64_t> tile_work;
+  gpairs.reserve(all_pairs.size());
+  std::vector<int64_t> out_work(outs.size());
+  int64_t ntiles = 0;
   double flops = 0, bytes = 0;
   for (size_t o = 0; o < outs.size(); ++o) {
     const Block& cb_ = out.blocks[o];
@@ -301,7 +312,8 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     gouts[o].ldc = int32_t(N);
     gouts[o].pair_begin = int32_t(gpairs.size());
     int64_t ksum = 0;
-    for (auto& p : outs[o].pairs) {
+    for (int q = outs[o].pb; q < outs[o].pe; ++q) {
+      const auto& p = all_pairs[size_t(q)];
       const View& x = va[p.first];
       const View& y = vb[p.second];
       kern::GemmPair gp{};
@@ -317,27 +329,29 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
       flops += 2.0 * double(M) * double(N) * double(x.K);
     }
     gouts[o].pair_end = int32_t(gpairs.size());
-    const int tm = int((M + kern::kGemmTile - 1) / kern::kGemmTile);
-    const int tn = int((N + kern::kGemmTile - 1) / kern::kGemmTile);
-    for (int i = 0; i < tm; ++i)
-      for (int j = 0; j < tn; ++j) {
-        tiles.push_back({int32_t(o), int16_t(i), int16_t(j)});
-        tile_work.push_back(ksum);
-      }
+    out_work[o] = ksum;
+    ntiles += ((M + kern::kGemmTile - 1) / kern::kGemmTile) * ((N + kern::kGemmTile - 1) / kern::kGemmTile);
     bytes += 8.0 * double(cb_.numel);
   }
   for (size_t k = 0; k < a.blocks.size(); ++k)
     if (used_a[k]) bytes += 8.0 * double(a.blocks[k].numel);
   for (size_t k = 0; k < b.blocks.size(); ++k)
     if (used_b[k]) bytes += 8.0 * double(b.blocks[k].numel);
-  // longest work first for a balanced tail
+  // longest work first for a balanced tail: the tiles of one output block
+  // share its summed K, so ordering the blocks (stably) by it and emitting
+  // each block's tiles row by row sorts the tiles by work
+  std::vector<kern::GemmTile> tiles;
+  tiles.reserve(size_t(ntiles));
   {
-    std::vector<int> ord(tiles.size());
+    std::vector<int> ord(outs.size());
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return tile_work[x] > tile_work[y]; });
-    std::vector<kern::GemmTile> st(tiles.size());
-    for (size_t i = 0; i < ord.size(); ++i) st[i] = tiles[ord[i]];
-    tiles.swap(st);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return out_work[x] > out_work[y]; });
+    for (int o : ord) {
+      const int tm = (gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
+      const int tn = (gouts[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
+      for (int i = 0; i < tm; ++i)
+        for (int j = 0; j < tn; ++j) tiles.push_back({int32_t(o), int16_t(i), int16_t(j)});
+    }
   }
   if (stats) {
     stats->flops += flops;
@@ -377,7 +391,10 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     }
     return out;
   }
+  ht_plan.stop();
+  HostTimer ht_up(*ctx, "host_td_upload");
   auto [d_tiles, d_outs, d_pairs] = ctx->upload_pack(tiles, gouts, gpairs);
+  ht_up.stop();
   if (ctx->precision == Precision::SPLIT_FP32) {
     KernelTimer kt(*ctx, "gemm_split");
     kern::gemm_split(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
