@@ -129,22 +129,28 @@ int env_int(const char* name, int dflt) {
 // rotates nothing (or max_sweeps).  first_rot[i] receives the number of
 // rotating work-groups of the first sweep of sector which[i].  Returns the
 // number of sweeps run.
+//
+// The sweeps are pipelined one deep: sweep s+1 is enqueued before the host
+// waits for the rotation counts of sweep s, so the device never idles on that
+// read.  Sweep s+1 is built from the counts up to sweep s-1; its work-groups
+// of a sector that sweep s left unrotated return at once (skip pointer), so
+// every sector stops after the same sweeps as with a read between sweeps.
 int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& secs, const std::vector<int>& which,
                   double tol, int inner, int max_sweeps, std::vector<int>* first_rot) {
   const int ns = int(secs.size());
-  if (which.empty()) return 0;
+  if (which.empty() || max_sweeps <= 0) return 0;
   auto* d_secs = ctx->upload_vec(secs);
-  BufPtr cnt = ctx->alloc(size_t(ns) * sizeof(int32_t));
-  int32_t* d_cnt = reinterpret_cast<int32_t*>(cnt->cptr());
+  BufPtr cnt = ctx->alloc(size_t(2 * ns) * sizeof(int32_t));
+  int32_t* d_cnt[2] = {reinterpret_cast<int32_t*>(cnt->cptr()), reinterpret_cast<int32_t*>(cnt->cptr()) + ns};
+  ctx->reserve_read(size_t(ns));
   std::vector<char> active(ns, 0);
   for (int i : which) active[i] = 1;
-  std::vector<int32_t> h_cnt(ns);
   if (first_rot) first_rot->assign(which.size(), 0);
   // From the second sweep on, rounds after the first of a sweep rotate only
   // the pairs across their two blocks (0: never, 1: from the first sweep).
   const int cross = env_int("QTT_SVD_CROSS_INNER", 2);
-  int sweeps = 0;
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+  // enqueues sweep `sweep` over the active sectors and the read of its counts
+  auto enqueue = [&](int sweep) {
     std::vector<int32_t> work;
     int max_nb = 0, max_ell = 0;
     for (int i = 0; i < ns; ++i) {
@@ -157,17 +163,27 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
         work.push_back(pi);
       }
     }
-    if (work.empty()) break;
-    kern::fill_i32(ctx->stream(), d_cnt, ns, 0);
+    if (work.empty()) return false;
+    const int slot = sweep & 1;
+    const int* skip = sweep > 0 ? d_cnt[slot ^ 1] : nullptr;
+    kern::fill_i32(ctx->stream(), d_cnt[slot], ns, 0);
     auto* d_work = ctx->upload_vec(work);
     for (int r = 0; r < max_nb - 1; ++r) {
       KernelTimer kt(*ctx, "svd_round");
-      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt, tol, inner, max_ell,
-                      cross == 1 || (cross == 2 && sweep > 0));
+      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt[slot], skip, tol, inner,
+                      max_ell, cross == 1 || (cross == 2 && sweep > 0));
     }
-    QTT_HIP(hipMemcpyAsync(h_cnt.data(), d_cnt, size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
+    QTT_HIP(hipMemcpyAsync(ctx->read_slot(slot), d_cnt[slot], size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
                            ctx->stream()));
-    ctx->sync();
+    QTT_HIP(hipEventRecord(ctx->read_event(slot), ctx->stream()));
+    return true;
+  };
+  if (!enqueue(0)) return 0;
+  int queued = 1, sweeps = 0;
+  for (int sweep = 0; sweep < queued; ++sweep) {
+    if (queued < max_sweeps && enqueue(queued)) ++queued;
+    QTT_HIP(hipEventSynchronize(ctx->read_event(sweep & 1)));
+    const int32_t* h_cnt = ctx->read_slot(sweep & 1);
     sweeps = sweep + 1;
     if (sweep == 0 && first_rot)
       for (size_t q = 0; q < which.size(); ++q) (*first_rot)[q] = h_cnt[which[q]];

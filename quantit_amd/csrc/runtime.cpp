@@ -1,5 +1,6 @@
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <cstdio>
 
 namespace qtt {
@@ -76,12 +77,16 @@ Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(d
   stage_cap_ = round_up(stage_bytes);
   QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_cap_, hipHostMallocDefault));
   dev_stage_ = alloc(stage_cap_);
+  for (auto& e : read_ev_) QTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 Context::~Context() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   pending_.clear();
   for (auto e : event_pool_) (void)hipEventDestroy(e);
+  for (auto e : read_ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (host_read_) (void)hipHostFree(host_read_);
   dev_stage_.reset();
   arena_.reset();
   if (host_stage_) (void)hipHostFree(host_stage_);
@@ -107,6 +112,17 @@ void* Context::upload(const void* data, size_t bytes) {
 }
 
 void Context::sync() { QTT_HIP(hipStreamSynchronize(stream_)); }
+
+void Context::reserve_read(size_t n_per_slot) {
+  if (n_per_slot > read_cap_) {
+    QTT_HIP(hipStreamSynchronize(stream_));  // no read may still target the old area
+    if (host_read_) QTT_HIP(hipHostFree(host_read_));
+    host_read_ = nullptr;
+    const size_t cap = std::max<size_t>(n_per_slot, 1024);
+    QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_read_), 2 * cap * sizeof(int32_t), hipHostMallocDefault));
+    read_cap_ = cap;
+  }
+}
 
 hipEvent_t Context::event() {
   if (event_next_ == event_pool_.size()) {

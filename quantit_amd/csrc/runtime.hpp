@@ -101,6 +101,14 @@ class Context {
   }
   void sync();
 
+  // Pinned host slots for small device-to-host reads that the host waits on
+  // with read_event(slot) while later work keeps the stream busy: slot 0 and
+  // slot 1 each hold n_per_slot 32-bit words (grown on demand; growing
+  // synchronizes the stream).
+  void reserve_read(size_t n_per_slot);
+  int32_t* read_slot(int slot) const { return host_read_ + size_t(slot) * read_cap_; }
+  hipEvent_t read_event(int slot) const { return read_ev_[slot]; }
+
   Precision precision = Precision::FP64;
   // Reference-shaped baseline: one GEMM launch per block pair (mm, addmm_).
   bool per_block_launch = false;
@@ -125,6 +133,9 @@ class Context {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending_;
+  int32_t* host_read_ = nullptr;
+  size_t read_cap_ = 0;  // words per slot
+  hipEvent_t read_ev_[2] = {nullptr, nullptr};
 };
 
 // RAII helper: brackets one launch with events when profiling is enabled.
