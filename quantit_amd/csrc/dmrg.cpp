@@ -149,12 +149,15 @@ BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
     kern::eig2x2(ctx->stream(), ab, a0.ptr(a0.blocks[0]) - ab, a1.ptr(a1.blocks[0]) - ab, b.ptr(b.blocks[0]) - ab,
                  trio->dptr() - ab, reinterpret_cast<int*>(flag->cptr()));
   }
-  int h_flag = 0;
-  QTT_HIP(hipMemcpyAsync(&h_flag, flag->cptr(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
+  // the flag lands in pinned memory without a wait; the truncation's read of
+  // its kept counts synchronizes the stream after it
+  int32_t* h_flag = ctx->pinned_flag();
+  *h_flag = 0;
+  QTT_HIP(hipMemcpyAsync(h_flag, flag->cptr(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
   BTensor theta2 = axpby_dev(theta, o, psi, n);
   SvdResult r = svd_truncated(theta2, 2, opt.cutoff, opt.minimum_bond, opt.maximum_bond, 2.0, &ss);
   ctx->sync();  // normally already done by the truncation; the flag copy is complete
-  if (h_flag) throw LogicError("nan found in output tensor");
+  if (*h_flag) throw LogicError("nan found in output tensor");
   normalize_d(r.d);
   if (st.step == 1) {
     st.mps[oc] = r.U;

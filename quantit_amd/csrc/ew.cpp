@@ -306,8 +306,7 @@ double item(const BTensor& s) {
   if (s.blocks.empty()) return 0.0;
   if (s.numel() != 1) throw LogicError("Only simgle block and single element tensors can be converted to scalar");
   double v = 0;
-  QTT_HIP(hipMemcpyAsync(&v, s.ptr(s.blocks[0]), sizeof(double), hipMemcpyDeviceToHost, s.ctx->stream()));
-  s.ctx->sync();
+  s.ctx->read_small(&v, s.ptr(s.blocks[0]), sizeof(double));
   return v;
 }
 

@@ -478,11 +478,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         kern::sum_parts(ctx->stream(), ab, ds, dst, int(st.size()));
       }
     };
-    auto read_flags = [&] {
-      QTT_HIP(hipMemcpyAsync(h_flags.data(), d_flags, size_t(2 * nw) * sizeof(int32_t), hipMemcpyDeviceToHost,
-                             ctx->stream()));
-      ctx->sync();
-    };
+    auto read_flags = [&] { ctx->read_small(h_flags.data(), d_flags, size_t(2 * nw) * sizeof(int32_t)); };
     auto retire = [&](std::vector<int>& qs) {  // drop converged sectors after a flag read
       std::vector<int> next;
       for (int q : qs) {
@@ -921,8 +917,7 @@ SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol,
                           int64_t(std::min<size_t>(max_size, size_t(clamp))), d_kept, d_thr);
   }
   std::vector<int32_t> kept(nd);
-  QTT_HIP(hipMemcpyAsync(kept.data(), d_kept, size_t(nd) * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream()));
-  ctx->sync();
+  ctx->read_small(kept.data(), d_kept, size_t(nd) * sizeof(int32_t));
 
   BTensor& U = in.U;
   BTensor& V = in.V;

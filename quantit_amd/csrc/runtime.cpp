@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 
 namespace qtt {
 
@@ -78,6 +79,7 @@ Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(d
   QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_cap_, hipHostMallocDefault));
   dev_stage_ = alloc(stage_cap_);
   for (auto& e : read_ev_) QTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_small_), kSmallBytes, hipHostMallocDefault));
 }
 
 Context::~Context() {
@@ -87,6 +89,7 @@ Context::~Context() {
   for (auto e : read_ev_)
     if (e) (void)hipEventDestroy(e);
   if (host_read_) (void)hipHostFree(host_read_);
+  if (host_small_) (void)hipHostFree(host_small_);
   dev_stage_.reset();
   arena_.reset();
   if (host_stage_) (void)hipHostFree(host_stage_);
@@ -112,6 +115,22 @@ void* Context::upload(const void* data, size_t bytes) {
 }
 
 void Context::sync() { QTT_HIP(hipStreamSynchronize(stream_)); }
+
+void Context::read_small(void* host_dst, const void* dev_src, size_t bytes) {
+  constexpr size_t kOff = 256;
+  char* dst = static_cast<char*>(host_dst);
+  const char* src = static_cast<const char*>(dev_src);
+  while (bytes) {
+    const size_t n = std::min(bytes, kSmallBytes - kOff);
+    char* h = reinterpret_cast<char*>(host_small_) + kOff;
+    QTT_HIP(hipMemcpyAsync(h, src, n, hipMemcpyDeviceToHost, stream_));
+    QTT_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(dst, h, n);
+    dst += n;
+    src += n;
+    bytes -= n;
+  }
+}
 
 void Context::reserve_read(size_t n_per_slot) {
   if (n_per_slot > read_cap_) {

@@ -105,6 +105,13 @@ class Context {
   // with read_event(slot) while later work keeps the stream busy: slot 0 and
   // slot 1 each hold n_per_slot 32-bit words (grown on demand; growing
   // synchronizes the stream).
+  // Device-to-host copy of a few bytes through pinned memory, waited for
+  // (the stream is drained up to the copy).  A copy into pageable memory
+  // would go through the runtime's own staging.
+  void read_small(void* host_dst, const void* dev_src, size_t bytes);
+  // A pinned word that a device-to-host copy may target without waiting;
+  // valid after the next synchronization of the stream.
+  int32_t* pinned_flag() { return host_small_; }
   void reserve_read(size_t n_per_slot);
   int32_t* read_slot(int slot) const { return host_read_ + size_t(slot) * read_cap_; }
   hipEvent_t read_event(int slot) const { return read_ev_[slot]; }
@@ -133,6 +140,8 @@ class Context {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending_;
+  int32_t* host_small_ = nullptr;  // word 0: pinned_flag; from byte 256: read_small
+  static constexpr size_t kSmallBytes = size_t(64) << 10;
   int32_t* host_read_ = nullptr;
   size_t read_cap_ = 0;  // words per slot
   hipEvent_t read_ev_[2] = {nullptr, nullptr};
