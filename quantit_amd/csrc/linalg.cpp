@@ -182,7 +182,7 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   int queued = 1, sweeps = 0;
   for (int sweep = 0; sweep < queued; ++sweep) {
     if (queued < max_sweeps && enqueue(queued)) ++queued;
-    QTT_HIP(hipEventSynchronize(ctx->read_event(sweep & 1)));
+    ctx->wait_event(ctx->read_event(sweep & 1));
     const int32_t* h_cnt = ctx->read_slot(sweep & 1);
     sweeps = sweep + 1;
     if (sweep == 0 && first_rot)

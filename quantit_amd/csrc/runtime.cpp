@@ -79,6 +79,7 @@ Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(d
   QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_cap_, hipHostMallocDefault));
   dev_stage_ = alloc(stage_cap_);
   for (auto& e : read_ev_) QTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  QTT_HIP(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
   QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_small_), kSmallBytes, hipHostMallocDefault));
 }
 
@@ -88,6 +89,7 @@ Context::~Context() {
   for (auto e : event_pool_) (void)hipEventDestroy(e);
   for (auto e : read_ev_)
     if (e) (void)hipEventDestroy(e);
+  if (sync_ev_) (void)hipEventDestroy(sync_ev_);
   if (host_read_) (void)hipHostFree(host_read_);
   if (host_small_) (void)hipHostFree(host_small_);
   dev_stage_.reset();
@@ -114,7 +116,21 @@ void* Context::upload(const void* data, size_t bytes) {
   return d;
 }
 
-void Context::sync() { QTT_HIP(hipStreamSynchronize(stream_)); }
+void Context::sync() {
+  QTT_HIP(hipEventRecord(sync_ev_, stream_));
+  wait_event(sync_ev_);
+}
+
+void Context::wait_event(hipEvent_t e) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t r = hipEventQuery(e);
+    if (r == hipSuccess) return;
+    if (r != hipErrorNotReady) QTT_HIP(r);
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+  }
+  QTT_HIP(hipEventSynchronize(e));
+}
 
 void Context::read_small(void* host_dst, const void* dev_src, size_t bytes) {
   constexpr size_t kOff = 256;
@@ -124,7 +140,7 @@ void Context::read_small(void* host_dst, const void* dev_src, size_t bytes) {
     const size_t n = std::min(bytes, kSmallBytes - kOff);
     char* h = reinterpret_cast<char*>(host_small_) + kOff;
     QTT_HIP(hipMemcpyAsync(h, src, n, hipMemcpyDeviceToHost, stream_));
-    QTT_HIP(hipStreamSynchronize(stream_));
+    sync();
     std::memcpy(dst, h, n);
     dst += n;
     src += n;

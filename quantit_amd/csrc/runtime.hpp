@@ -100,6 +100,9 @@ class Context {
     return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
   }
   void sync();
+  // Waits for an event recorded on this stream: polls it for up to 2 ms (a
+  // short wait then ends without a sleep and wake-up), then blocks.
+  void wait_event(hipEvent_t e);
 
   // Pinned host slots for small device-to-host reads that the host waits on
   // with read_event(slot) while later work keeps the stream busy: slot 0 and
@@ -145,6 +148,7 @@ class Context {
   int32_t* host_read_ = nullptr;
   size_t read_cap_ = 0;  // words per slot
   hipEvent_t read_ev_[2] = {nullptr, nullptr};
+  hipEvent_t sync_ev_ = nullptr;
 };
 
 // RAII helper: brackets one launch with events when profiling is enabled.
