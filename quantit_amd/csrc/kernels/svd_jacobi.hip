@@ -179,11 +179,13 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
       double* dst = Yl + c * pitch;
       for (int o = 0; o < ell; o += 32 * CH) {
         double v[CH];
+        // 32-row slices wholly past ell are skipped (a uniform branch): their
+        // clamped loads would only occupy the address units
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
           const int r = o + part + 32 * u;
-          const double x = src[r < ell ? r : ell - 1];
-          v[u] = r < ell ? x : 0.0;
+          v[u] = 0.0;
+          if (o + 32 * u < ell) v[u] = src[r < ell ? r : ell - 1];
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
@@ -193,36 +195,44 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
       }
     }
     __syncthreads();
-    // sixteen waves: four per 16x16 tile, each on the 4-row slices 4 qtr +
-    // 16 t, two accumulation chains each; the quarters are summed into G in
-    // a fixed order below
+    // G is symmetric: the three distinct 16x16 tiles (0,0), (0,1), (1,1),
+    // five waves each on the 4-row slices 4 part + 20 t (two accumulation
+    // chains); the fifths are summed into G in a fixed order below and the
+    // (0,1) tile is mirrored into (1,0).  Wave 15 only meets the barriers.
     const int cl = lane & 15, kq = lane >> 4;
-    const int tile = wave & 3, qtr = wave >> 2;
-    const int ti = tile >> 1, tj = tile & 1;
+    constexpr int kParts = 5;
+    const bool gw = wave < 3 * kParts;
+    const int tile = gw ? wave % 3 : 0, part = gw ? wave / 3 : kParts;
+    const int ti = tile == 2 ? 1 : 0, tj = tile == 0 ? 0 : 1;
     const double* ya = Yl + (ti * 16 + cl) * pitch + kq;
     const double* yb = Yl + (tj * 16 + cl) * pitch + kq;
     v4d g1 = v4d{0.0, 0.0, 0.0, 0.0};
-    int kk = 4 * qtr;
-    for (; kk + 16 < ell; kk += 32) {
-      const bool in1 = kk + 16 + kq < ell;
-      const double a0 = ya[kk], b0 = yb[kk];
-      const double a1 = ya[in1 ? kk + 16 : 0], b1 = yb[in1 ? kk + 16 : 0];
-      g = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, g, 0, 0, 0);
-      g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(in1 ? a1 : 0.0, in1 ? b1 : 0.0, g1, 0, 0, 0);
+    if (gw) {
+      int kk = 4 * part;
+      for (; kk + 4 * kParts < ell; kk += 8 * kParts) {
+        const bool in1 = kk + 4 * kParts + kq < ell;
+        const double a0 = ya[kk], b0 = yb[kk];
+        const double a1 = ya[in1 ? kk + 4 * kParts : 0], b1 = yb[in1 ? kk + 4 * kParts : 0];
+        g = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, g, 0, 0, 0);
+        g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(in1 ? a1 : 0.0, in1 ? b1 : 0.0, g1, 0, 0, 0);
+      }
+      for (; kk < ell; kk += 4 * kParts) {
+        const bool in = kk + kq < ell;
+        const double xa = ya[in ? kk : 0], xb = yb[in ? kk : 0];
+        g = __builtin_amdgcn_mfma_f64_16x16x4f64(in ? xa : 0.0, in ? xb : 0.0, g, 0, 0, 0);
+      }
+      g = g + g1;
     }
-    for (; kk < ell; kk += 16) {
-      const bool in = kk + kq < ell;
-      const double xa = ya[in ? kk : 0], xb = yb[in ? kk : 0];
-      g = __builtin_amdgcn_mfma_f64_16x16x4f64(in ? xa : 0.0, in ? xb : 0.0, g, 0, 0, 0);
-    }
-    g = g + g1;
 #pragma unroll
-    for (int qq = 0; qq < 4; ++qq) {
-      if (qtr == qq) {
+    for (int qq = 0; qq < kParts; ++qq) {
+      if (part == qq) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          double& dst = G[0][ti * 16 + 4 * r + (lane >> 4)][tj * 16 + (lane & 15)];
-          dst = qq == 0 ? g[r] : dst + g[r];
+          const int gi = ti * 16 + 4 * r + (lane >> 4), gj = tj * 16 + (lane & 15);
+          double& dst = G[0][gi][gj];
+          const double v = qq == 0 ? g[r] : dst + g[r];
+          dst = v;
+          if (tile == 1) G[0][gj][gi] = v;
         }
       }
       __syncthreads();
@@ -327,8 +337,18 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
         int a, a2, b, b2;
         pair_of(P, r, a, a2);
         pair_of(R, r, b, b2);
-        const double g00 = G[0][a][b], g01 = G[0][a][b2], g10 = G[0][a2][b], g11 = G[0][a2][b2];
-        const double q00 = Q[0][a][b], q01 = Q[0][a][b2], q10 = Q[0][a2][b], q11 = Q[0][a2][b2];
+        // only the block owners read (the other waves would multiply the LDS traffic)
+        double g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0, q00 = 0.0, q01 = 0.0, q10 = 0.0, q11 = 0.0;
+        if (blk) {
+          g00 = G[0][a][b];
+          g01 = G[0][a][b2];
+          g10 = G[0][a2][b];
+          g11 = G[0][a2][b2];
+          q00 = Q[0][a][b];
+          q01 = Q[0][a][b2];
+          q10 = Q[0][a2][b];
+          q11 = Q[0][a2][b2];
+        }
         __syncthreads();  // rotations published; every read of this step precedes the writes
         if (blk) {
           const double cp = rc[P], sp = rs[P], cq = rc[R], sq = rs[R];
