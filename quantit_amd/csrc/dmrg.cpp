@@ -23,13 +23,25 @@ Shape edge_shape_prep(const Shape& t, int dim) {
   return shape_product({&Si, &s0});
 }
 
+namespace {
+// Labels the contraction GEMMs of a call site in the profiling timers.
+struct GemmLabel {
+  Context* ctx;
+  const char* prev;
+  GemmLabel(Context* c, const char* name) : ctx(c), prev(c->gemm_label) { ctx->gemm_label = name; }
+  ~GemmLabel() { ctx->gemm_label = prev; }
+};
+}  // namespace
+
 BTensor compute_left_env(const BTensor& W, const BTensor& Y, const BTensor& L) {
+  GemmLabel gl(L.ctx, "gemm_env");
   BTensor out = tensordot(L, Y, {0}, {0});
   out = tensordot(out, W, {0, 2}, {0, 3});
   return tensordot(out, conj(Y), {0, 2}, {0, 1});
 }
 
 BTensor compute_right_env(const BTensor& W, const BTensor& Y, const BTensor& R) {
+  GemmLabel gl(R.ctx, "gemm_env");
   BTensor out = tensordot(R, Y, {0}, {2});
   out = tensordot(out, W, {0, 3}, {2, 3});
   return tensordot(out, conj(Y), {3, 0}, {1, 2});
@@ -116,8 +128,16 @@ BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
   BTensor theta = tensordot(st.mps[oc], st.mps[oc + 1], {2}, {0}, &cs);
   // one_step_lanczos (dmrg.cpp:584-608)
   auto matvec = [&](const BTensor& x) {
-    BTensor out = tensordot(Lenv, x, {0}, {0}, &cs);
-    out = tensordot(out, W, {0, 2, 3}, {0, 4, 5}, &cs);
+    BTensor out;
+    {
+      GemmLabel gl(ctx, "gemm_mv_L");
+      out = tensordot(Lenv, x, {0}, {0}, &cs);
+    }
+    {
+      GemmLabel gl(ctx, "gemm_mv_W");
+      out = tensordot(out, W, {0, 2, 3}, {0, 4, 5}, &cs);
+    }
+    GemmLabel gl(ctx, "gemm_mv_R");
     return tensordot(out, Renv, {1, 4}, {0, 1}, &cs);
   };
   BTensor psi = matvec(theta);

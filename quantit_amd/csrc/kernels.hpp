@@ -40,6 +40,16 @@ void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, co
 // chunk and are folded into FP64 accumulators after every chunk.
 void gemm_split(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
                 const GemmPair* pairs, double alpha, double beta);
+// Output blocks with at most kRowsMaxN columns and a short summed K: one
+// thread per output row keeps the N accumulators in registers and walks the
+// pairs in order, k ascending (one FP64 FMA chain per element).  A task is
+// kRowsPerTask consecutive rows of one output block.
+constexpr int kRowsMaxN = 16, kRowsMaxK = 128, kRowsPerTask = 256;
+struct RowTask {
+  int32_t out, row0;
+};
+void gemm_rows(hipStream_t s, double* base, const RowTask* tasks, int ntasks, const GemmOut* outs,
+               const GemmPair* pairs, double alpha, double beta);
 
 // ---- data movement ----------------------------------------------------------
 // Row-major copy of a permuted view: dst element with multi-index j over

@@ -303,6 +303,50 @@ __global__ __launch_bounds__(256) void k_gemm_split(double* __restrict__ base, c
   store_tile<false>(base + o.c_off, o, m0, n0, wm, wn, lane, acc, alpha, beta);
 }
 
+// Narrow outputs (N <= kRowsMaxN, summed K <= kRowsMaxK): an MFMA tile
+// would be mostly padding there.  The pair descriptors and B are uniform
+// across the work-group (scalar loads); A is read along rows, so lanes read
+// consecutive rows when its row stride is 1.
+__global__ __launch_bounds__(kRowsPerTask) void k_gemm_rows(double* __restrict__ base,
+                                                             const RowTask* __restrict__ tasks,
+                                                             const GemmOut* __restrict__ outs,
+                                                             const GemmPair* __restrict__ pairs, double alpha,
+                                                             double beta) {
+  const RowTask t = tasks[blockIdx.x];
+  const GemmOut o = outs[t.out];
+  const int m = t.row0 + int(threadIdx.x);
+  if (m >= o.M) return;
+  double acc[kRowsMaxN];
+#pragma unroll
+  for (int n = 0; n < kRowsMaxN; ++n) acc[n] = 0.0;
+  for (int p = o.pair_begin; p < o.pair_end; ++p) {
+    const GemmPair pr = pairs[p];
+    const double* A = base + pr.a_off + int64_t(m) * pr.a_rs;
+    const double* B = base + pr.b_off;
+    for (int k = 0; k < pr.K; ++k) {
+      const double a = A[int64_t(k) * pr.a_cs];
+      const double* Bk = B + int64_t(k) * pr.b_rs;
+#pragma unroll
+      for (int n = 0; n < kRowsMaxN; ++n)
+        if (n < o.N) acc[n] = fma(a, Bk[int64_t(n) * pr.b_cs], acc[n]);
+    }
+  }
+  double* C = base + o.c_off + int64_t(m) * o.ldc;
+#pragma unroll
+  for (int n = 0; n < kRowsMaxN; ++n)
+    if (n < o.N) {
+      double v = alpha * acc[n];
+      if (beta != 0.0) v += beta * C[n];
+      C[n] = v;
+    }
+}
+
+void gemm_rows(hipStream_t s, double* base, const RowTask* tasks, int ntasks, const GemmOut* outs,
+               const GemmPair* pairs, double alpha, double beta) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_gemm_rows, dim3(ntasks), dim3(kRowsPerTask), 0, s, base, tasks, outs, pairs, alpha, beta);
+}
+
 void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
               const GemmPair* pairs, double alpha, double beta) {
   if (ntiles <= 0) return;

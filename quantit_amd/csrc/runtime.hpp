@@ -124,6 +124,9 @@ class Context {
   bool per_block_launch = false;
   // Per-kernel accumulated time is recorded when profiling is on.
   bool profiling = false;
+  // Timer name of the contraction GEMMs (callers label their call sites;
+  // null: "gemm_f64" / "gemm_split").
+  const char* gemm_label = nullptr;
   std::map<std::string, std::pair<double, long>> kernel_ms;  // name -> (ms, launches)
   void record(const char* name, hipEvent_t a, hipEvent_t b);
   hipEvent_t event();

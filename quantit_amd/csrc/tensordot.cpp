@@ -2,6 +2,7 @@
 // output block list and accumulation order of btensor::tensordot
 // (sources/btensor.cpp:1971-2119), and one grouped-GEMM launch on device.
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 
 #include "btensor.hpp"
@@ -339,14 +340,24 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     if (used_b[k]) bytes += 8.0 * double(b.blocks[k].numel);
   // longest work first for a balanced tail: the tiles of one output block
   // share its summed K, so ordering the blocks (stably) by it and emitting
-  // each block's tiles row by row sorts the tiles by work
+  // each block's tiles row by row sorts the tiles by work.  Narrow blocks
+  // (few columns, short summed K: an MFMA tile would be mostly padding) go
+  // to the row kernel instead (FP64 mode).
   std::vector<kern::GemmTile> tiles;
+  std::vector<kern::RowTask> row_tasks;
   tiles.reserve(size_t(ntiles));
   {
+    const char* rows_env = std::getenv("QTT_GEMM_ROWS");  // "0": MFMA tiles for every block
+    const bool rows_ok = ctx->precision == Precision::FP64 && !ctx->per_block_launch &&
+                         !(rows_env && std::atoi(rows_env) == 0);
     std::vector<int> ord(outs.size());
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return out_work[x] > out_work[y]; });
     for (int o : ord) {
+      if (rows_ok && gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK) {
+        for (int r0 = 0; r0 < gouts[o].M; r0 += kern::kRowsPerTask) row_tasks.push_back({int32_t(o), int32_t(r0)});
+        continue;
+      }
       const int tm = (gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
       const int tn = (gouts[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
       for (int i = 0; i < tm; ++i)
@@ -393,13 +404,17 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   }
   ht_plan.stop();
   HostTimer ht_up(*ctx, "host_td_upload");
-  auto [d_tiles, d_outs, d_pairs] = ctx->upload_pack(tiles, gouts, gpairs);
+  auto [d_tiles, d_outs, d_pairs, d_rows] = ctx->upload_pack(tiles, gouts, gpairs, row_tasks);
   ht_up.stop();
+  if (!row_tasks.empty()) {
+    KernelTimer kt(*ctx, "gemm_rows");
+    kern::gemm_rows(ctx->stream(), ab, d_rows, int(row_tasks.size()), d_outs, d_pairs, 1.0, 0.0);
+  }
   if (ctx->precision == Precision::SPLIT_FP32) {
-    KernelTimer kt(*ctx, "gemm_split");
+    KernelTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_split");
     kern::gemm_split(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
   } else {
-    KernelTimer kt(*ctx, "gemm_f64");
+    KernelTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_f64");
     kern::gemm_f64(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
   }
   return out;

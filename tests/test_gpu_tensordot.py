@@ -59,6 +59,34 @@ def test_tensordot_matches_oracle(ctx, case, groups):
     assert np.allclose(out.to_dense(), dense, atol=1e-11 * max(1.0, np.abs(dense).max()))
 
 
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_tensordot_narrow_rows_equal_mfma_tiles(ctx, case, monkeypatch):
+    """Narrow output blocks (few columns, short summed K) take the row kernel;
+    the same contraction with every block on MFMA tiles agrees to 1e-13, and
+    the row kernel is seen to run."""
+    (ra, rb), da, db = CASES[case]
+    rng = np.random.default_rng(300 + case)
+    A, B = _pair(rng, G1, ra, rb, da, db, nsec=3 if ra + rb > 8 else 4)
+    a, b = to_dev(ctx, A), to_dev(ctx, B)
+    ctx.kernel_times(reset=True)
+    ctx.set_profiling(True)
+    try:
+        rows = from_dev(a.tensordot(b, da, db))
+    finally:
+        ctx.set_profiling(False)
+    kt = ctx.kernel_times(reset=True)
+    monkeypatch.setenv("QTT_GEMM_ROWS", "0")
+    tiles = from_dev(a.tensordot(b, da, db))
+    assert_same_structure(rows, tiles)
+    assert_values_close(rows, tiles, 1e-13)
+    ref = ref_tensordot(A, B, da, db)
+    assert_values_close(rows, ref, 1e-12)
+    # with one free leg of B (sections of at most 5 columns) every output
+    # block of these seeded cases is narrow, so the row kernel must have run
+    if rb - len(db) == 1 and rows.blocks:
+        assert "gemm_rows" in kt
+
+
 def test_tensordot_large_blocks(ctx):
     """Blocks larger than one 64x64 tile and K spanning several LDS stages."""
     rng = np.random.default_rng(7)
