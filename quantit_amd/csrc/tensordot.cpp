@@ -355,7 +355,8 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return out_work[x] > out_work[y]; });
     for (int o : ord) {
       if (rows_ok && gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK) {
-        for (int r0 = 0; r0 < gouts[o].M; r0 += kern::kRowsPerTask) row_tasks.push_back({int32_t(o), int32_t(r0)});
+        const int rpt = kern::rows_per_task(gouts[o].N);
+        for (int r0 = 0; r0 < gouts[o].M; r0 += rpt) row_tasks.push_back({int32_t(o), int32_t(r0)});
         continue;
       }
       const int tm = (gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
