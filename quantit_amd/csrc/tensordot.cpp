@@ -2,6 +2,7 @@
 // output block list and accumulation order of btensor::tensordot
 // (sources/btensor.cpp:1971-2119), and one grouped-GEMM launch on device.
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <numeric>
 
@@ -338,29 +339,114 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     if (used_a[k]) bytes += 8.0 * double(a.blocks[k].numel);
   for (size_t k = 0; k < b.blocks.size(); ++k)
     if (used_b[k]) bytes += 8.0 * double(b.blocks[k].numel);
-  // longest work first for a balanced tail: the tiles of one output block
-  // share its summed K, so ordering the blocks (stably) by it and emitting
-  // each block's tiles row by row sorts the tiles by work.  Narrow blocks
-  // (few columns, short summed K: an MFMA tile would be mostly padding) go
-  // to the row kernel instead (FP64 mode).
+  // Narrow blocks (few columns, short summed K: an MFMA tile would be mostly
+  // padding) go to the row kernel (FP64 mode); the others to MFMA tiles.
+  const char* rows_env = std::getenv("QTT_GEMM_ROWS");  // "0": MFMA tiles for every block
+  const bool rows_ok = ctx->precision == Precision::FP64 && !ctx->per_block_launch &&
+                       !(rows_env && std::atoi(rows_env) == 0);
+  std::vector<char> narrow(outs.size(), 0);
+  int64_t mfma_tiles = 0;
+  for (size_t o = 0; o < outs.size(); ++o) {
+    narrow[o] = rows_ok && gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK;
+    if (!narrow[o])
+      mfma_tiles += int64_t((gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile) *
+                    ((gouts[o].N + kern::kGemmTile - 1) / kern::kGemmTile);
+  }
+  // Split-K when the MFMA tiles would leave most CUs idle (long-K
+  // contractions with few output tiles, such as the right environment of
+  // the matvec): the K extent of a block, its pairs in order, is cut into
+  // about equal runs (a pair may be cut: a sub-pair is the same pair with A
+  // and B advanced by k0 and a shorter K); each run accumulates into a
+  // partial of its own and a fixed-order sum writes the block.  launch_outs
+  // = gouts followed by the runs; split blocks get no tiles of their own.
+  std::vector<kern::GemmOut> launch_outs = gouts;
+  std::vector<int64_t> launch_work = out_work;
+  std::vector<char> split_block(outs.size(), 0);
+  std::vector<kern::SumParts> sums;
+  std::vector<kern::ChunkTask> sum_tasks;
+  BufPtr partials;
+  {
+    constexpr int64_t kTargetTiles = 1024, kSplitMinK = 128;
+    constexpr int64_t kMaxSplit = 8;
+    const char* split_env = std::getenv("QTT_GEMM_SPLIT_K");  // "0": never split
+    const bool split_ok = !ctx->per_block_launch && !(split_env && std::atoi(split_env) == 0);
+    const int64_t split = split_ok ? std::min(kMaxSplit, kTargetTiles / std::max<int64_t>(mfma_tiles, 1)) : 1;
+    std::vector<int> nparts(outs.size(), 0);
+    int64_t part_elems = 0;
+    if (split >= 2)
+      for (size_t o = 0; o < outs.size(); ++o) {
+        if (narrow[o]) continue;
+        const int64_t np = std::min(split, out_work[o] / kSplitMinK);
+        if (np < 2) continue;
+        nparts[o] = int(np);
+        part_elems += np * gouts[o].M * gouts[o].N;
+      }
+    if (part_elems) {
+      partials = ctx->alloc(size_t(part_elems) * sizeof(double));
+      int64_t pos = partials->dptr() - ab;
+      for (size_t o = 0; o < outs.size(); ++o) {
+        const int np = nparts[o];
+        if (!np) continue;
+        split_block[o] = 1;
+        const int64_t mn = int64_t(gouts[o].M) * gouts[o].N;
+        add_chunks(sum_tasks, int(sums.size()), mn);
+        sums.push_back({gouts[o].c_off, pos, mn, int32_t(np), 0});
+        // runs of about ksum / np, rounded up to whole k-tiles
+        const int64_t run = ((out_work[o] + np - 1) / np + 15) / 16 * 16;
+        int q = gouts[o].pair_begin;
+        int64_t k0 = 0;
+        for (int part = 0; part < np; ++part) {
+          kern::GemmOut g = gouts[o];
+          g.c_off = pos + int64_t(part) * mn;
+          g.ldc = g.N;
+          g.pair_begin = int32_t(gpairs.size());
+          int64_t left = part + 1 == np ? INT64_MAX : run;
+          while (left > 0 && q < gouts[o].pair_end) {
+            const kern::GemmPair src = gpairs[size_t(q)];  // a copy: push_back below may reallocate
+            const int64_t take = std::min<int64_t>(left, src.K - k0);
+            if (take > 0) {
+              kern::GemmPair sp = src;
+              sp.a_off += k0 * src.a_cs;
+              sp.b_off += k0 * src.b_rs;
+              sp.K = int32_t(take);
+              gpairs.push_back(sp);
+              left -= take;
+              k0 += take;
+            }
+            if (k0 >= src.K) {
+              ++q;
+              k0 = 0;
+            }
+          }
+          g.pair_end = int32_t(gpairs.size());
+          int64_t kw = 0;
+          for (int r = g.pair_begin; r < g.pair_end; ++r) kw += gpairs[size_t(r)].K;
+          launch_outs.push_back(g);
+          launch_work.push_back(kw);
+        }
+        pos += int64_t(np) * mn;
+      }
+    }
+  }
+  // longest work first for a balanced tail: the tiles of one output share
+  // its summed K, so ordering the outputs (stably) by it and emitting each
+  // one's tiles row by row sorts the tiles by work
   std::vector<kern::GemmTile> tiles;
   std::vector<kern::RowTask> row_tasks;
-  tiles.reserve(size_t(ntiles));
+  tiles.reserve(size_t(mfma_tiles));
   {
-    const char* rows_env = std::getenv("QTT_GEMM_ROWS");  // "0": MFMA tiles for every block
-    const bool rows_ok = ctx->precision == Precision::FP64 && !ctx->per_block_launch &&
-                         !(rows_env && std::atoi(rows_env) == 0);
-    std::vector<int> ord(outs.size());
+    std::vector<int> ord(launch_outs.size());
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return out_work[x] > out_work[y]; });
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return launch_work[x] > launch_work[y]; });
     for (int o : ord) {
-      if (rows_ok && gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK) {
+      if (size_t(o) < outs.size() && narrow[size_t(o)]) {
         const int rpt = kern::rows_per_task(gouts[o].N);
         for (int r0 = 0; r0 < gouts[o].M; r0 += rpt) row_tasks.push_back({int32_t(o), int32_t(r0)});
         continue;
       }
-      const int tm = (gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
-      const int tn = (gouts[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
+      if (size_t(o) < outs.size() && split_block[size_t(o)]) continue;
+      const int tm = (launch_outs[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
+      const int tn = (launch_outs[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
       for (int i = 0; i < tm; ++i)
         for (int j = 0; j < tn; ++j) tiles.push_back({int32_t(o), int16_t(i), int16_t(j)});
     }
@@ -405,7 +491,8 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   }
   ht_plan.stop();
   HostTimer ht_up(*ctx, "host_td_upload");
-  auto [d_tiles, d_outs, d_pairs, d_rows] = ctx->upload_pack(tiles, gouts, gpairs, row_tasks);
+  auto [d_tiles, d_outs, d_pairs, d_rows, d_sums, d_sum_tasks] =
+      ctx->upload_pack(tiles, launch_outs, gpairs, row_tasks, sums, sum_tasks);
   ht_up.stop();
   if (!row_tasks.empty()) {
     KernelTimer kt(*ctx, "gemm_rows");
@@ -417,6 +504,10 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   } else {
     KernelTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_f64");
     kern::gemm_f64(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
+  }
+  if (!sums.empty()) {
+    KernelTimer kt(*ctx, "gemm_split_k_sum");
+    kern::sum_parts(ctx->stream(), ab, d_sums, d_sum_tasks, int(sum_tasks.size()));
   }
   return out;
 }

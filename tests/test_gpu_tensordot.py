@@ -87,6 +87,45 @@ def test_tensordot_narrow_rows_equal_mfma_tiles(ctx, case, monkeypatch):
         assert "gemm_rows" in kt
 
 
+@pytest.mark.parametrize("shape", ["many_pairs", "one_long_pair"])
+def test_tensordot_split_k(ctx, shape, monkeypatch):
+    """Few output tiles and a long contracted extent: the K range of each
+    block (pairs in order, a pair possibly cut) is split into partials that
+    a fixed-order sum combines.  Values against the oracle and against the
+    unsplit launch, the split seen to run, two runs bit-identical."""
+    rng = np.random.default_rng(12)
+    la = rand_leg(rng, 3, lo=-3, hi=3, smin=17, smax=40, groups=G1)
+    if shape == "many_pairs":
+        lx = rand_leg(rng, 7, lo=-3, hi=3, smax=40, groups=G1)
+        ly = rand_leg(rng, 7, lo=-3, hi=3, smax=40, groups=G1)
+        A = rand_btensor(rng, G1, [la, lx, ly], (0,))
+        B = rand_btensor(rng, G1, [inv_leg(G1, lx), inv_leg(G1, ly), inv_leg(G1, la)], (0,))
+        da, db = [1, 2], [0, 1]
+    else:  # one contracted sector of 700: a single pair per output block
+        lx = [(700, (0,))]
+        A = rand_btensor(rng, G1, [la, lx, inv_leg(G1, la)], (0,))
+        B = rand_btensor(rng, G1, [inv_leg(G1, lx), [(30, (0,))]], (0,))
+        da, db = [1], [0]
+    ref = ref_tensordot(A, B, da, db)
+    a, b = to_dev(ctx, A), to_dev(ctx, B)
+    ctx.kernel_times(reset=True)
+    ctx.set_profiling(True)
+    try:
+        x = from_dev(a.tensordot(b, da, db))
+        y = from_dev(a.tensordot(b, da, db))
+    finally:
+        ctx.set_profiling(False)
+    kt = ctx.kernel_times(reset=True)
+    assert "gemm_split_k_sum" in kt
+    assert_same_structure(x, ref)
+    assert_values_close(x, ref, 1e-12)
+    for k in x.blocks:
+        assert np.array_equal(x.blocks[k], y.blocks[k])
+    monkeypatch.setenv("QTT_GEMM_SPLIT_K", "0")
+    whole = from_dev(a.tensordot(b, da, db))
+    assert_values_close(x, whole, 1e-13)
+
+
 def test_tensordot_large_blocks(ctx):
     """Blocks larger than one 64x64 tile and K spanning several LDS stages."""
     rng = np.random.default_rng(7)
