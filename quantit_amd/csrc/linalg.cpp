@@ -601,7 +601,12 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       gram_of(act);
       std::vector<int> act_s, act_l;
       for (int q : act) (large[wide[q]] ? act_l : act_s).push_back(q);
-      // small sectors: test and solve in one work-group each
+      // small sectors: test and solve in one work-group each.  Beside large
+      // sectors they run on the side stream, concurrently with the large
+      // sectors' Cholesky and rounds (which leave most CUs free); the main
+      // stream joins before the rotation.  Profiling keeps one stream so that
+      // the event timers stay per kernel.
+      const bool overlap = !act_s.empty() && !act_l.empty() && !ctx->profiling;
       if (!act_s.empty()) {
         std::vector<kern::SmallJacobi> jd;
         for (int q : act_s) {
@@ -609,9 +614,11 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
           jd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, q, nw + q});
         }
         static const char* const kEigNames[] = {"svd_eig", "svd_eig_it1", "svd_eig_it2", "svd_eig_it3+"};
+        const auto* d_jd = ctx->upload_vec(jd);
+        if (overlap) ctx->fork();
         KernelTimer kt(*ctx, kEigNames[std::min(it, 3)]);
-        kern::gram_eig(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), d_flags, tol, max_sweeps,
-                       env_int("QTT_SVD_PRECOND", 1), lds_need);
+        kern::gram_eig(overlap ? ctx->side_stream() : ctx->stream(), ab, d_jd, int(jd.size()), d_flags, tol,
+                       max_sweeps, env_int("QTT_SVD_PRECOND", 1), lds_need);
       }
       // large sectors: test first (the solve is a long chain of launches); the
       // flag read also covers the small solver launched above
@@ -629,12 +636,13 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
           KernelTimer kt(*ctx, "svd_check");
           kern::gram_check(ctx->stream(), ab, ctx->upload_vec(cd), int(cd.size()), d_flags, tol, kc);
         }
-        read_flags();
+        read_flags();  // the small sectors' flags may not be written yet when overlapped
         retire(act_l);
         solve_large(act_l);
       }
+      if (overlap) ctx->join();
       if (it >= 1) {  // the small solver's test of this iteration's G
-        if (!act_s.empty() && act_l.empty() && cd_launched == 0) read_flags();
+        if (!act_s.empty() && ((act_l.empty() && cd_launched == 0) || overlap)) read_flags();
         retire(act_s);
       }
       act = act_s;

@@ -74,6 +74,9 @@ void Arena::release(size_t off) {
 Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(device) {
   QTT_HIP(hipSetDevice(device));
   QTT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  QTT_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+  QTT_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+  QTT_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
   arena_ = std::make_unique<Arena>(arena_bytes);
   stage_cap_ = round_up(stage_bytes);
   QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_cap_, hipHostMallocDefault));
@@ -85,7 +88,10 @@ Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(d
 
 Context::~Context() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (side_) (void)hipStreamSynchronize(side_);
   pending_.clear();
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  if (join_ev_) (void)hipEventDestroy(join_ev_);
   for (auto e : event_pool_) (void)hipEventDestroy(e);
   for (auto e : read_ev_)
     if (e) (void)hipEventDestroy(e);
@@ -95,6 +101,7 @@ Context::~Context() {
   dev_stage_.reset();
   arena_.reset();
   if (host_stage_) (void)hipHostFree(host_stage_);
+  if (side_) (void)hipStreamDestroy(side_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -102,8 +109,10 @@ void* Context::upload(const void* data, size_t bytes) {
   size_t need = round_up(bytes ? bytes : 1);
   if (need > stage_cap_) throw OutOfMemory("descriptor upload larger than the staging ring");
   if (stage_pos_ + need > stage_cap_) {
-    // wrap: everything previously staged must have been consumed
+    // wrap: everything previously staged must have been consumed, by the
+    // work of both streams
     QTT_HIP(hipStreamSynchronize(stream_));
+    QTT_HIP(hipStreamSynchronize(side_));
     stage_pos_ = 0;
   }
   char* h = host_stage_ + stage_pos_;
@@ -114,6 +123,16 @@ void* Context::upload(const void* data, size_t bytes) {
   }
   stage_pos_ += need;
   return d;
+}
+
+void Context::fork() {
+  QTT_HIP(hipEventRecord(fork_ev_, stream_));
+  QTT_HIP(hipStreamWaitEvent(side_, fork_ev_, 0));
+}
+
+void Context::join() {
+  QTT_HIP(hipEventRecord(join_ev_, side_));
+  QTT_HIP(hipStreamWaitEvent(stream_, join_ev_, 0));
 }
 
 void Context::sync() {

@@ -70,6 +70,13 @@ class Context {
 
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
+  // A second stream for independent work beside the main one: fork() makes
+  // it wait for everything enqueued on the main stream so far, join() makes
+  // the main stream wait for everything enqueued on it.  Both are device-side
+  // waits (no host synchronization).
+  hipStream_t side_stream() const { return side_; }
+  void fork();
+  void join();
   Arena& arena() { return *arena_; }
   BufPtr alloc(size_t bytes) { return std::make_shared<DeviceBuffer>(arena_.get(), bytes); }
 
@@ -135,6 +142,8 @@ class Context {
  private:
   int device_;
   hipStream_t stream_ = nullptr;
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
   std::unique_ptr<Arena> arena_;
   char* host_stage_ = nullptr;
   BufPtr dev_stage_;
