@@ -130,11 +130,14 @@ int env_int(const char* name, int dflt) {
 // rotating work-groups of the first sweep of sector which[i].  Returns the
 // number of sweeps run.
 //
-// The sweeps are pipelined one deep: sweep s+1 is enqueued before the host
-// waits for the rotation counts of sweep s, so the device never idles on that
-// read.  Sweep s+1 is built from the counts up to sweep s-1; its work-groups
-// of a sector that sweep s left unrotated return at once (skip pointer), so
-// every sector stops after the same sweeps as with a read between sweeps.
+// The sweeps are pipelined one deep while they rotate much: sweep s+1 is
+// enqueued before the host waits for the rotation counts of sweep s, so the
+// device does not idle on that read.  Sweep s+1 is built from the counts up
+// to sweep s-1; its work-groups of a sector that sweep s left unrotated
+// return at once (skip pointer), so every sector stops after the same sweeps
+// as with a read between sweeps.  Once a sweep rotates under a quarter of its
+// pair-rounds the end is near and the next sweep is enqueued only after the
+// read (a speculative sweep after the last would be launches of no-ops).
 int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& secs, const std::vector<int>& which,
                   double tol, int inner, int max_sweeps, std::vector<int>* first_rot) {
   const int ns = int(secs.size());
@@ -150,6 +153,7 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   // the pairs across their two blocks (0: never, 1: from the first sweep).
   const int cross = env_int("QTT_SVD_CROSS_INNER", 2);
   // enqueues sweep `sweep` over the active sectors and the read of its counts
+  std::vector<int64_t> slot_rounds(2, 0);  // pair-rounds enqueued per slot
   auto enqueue = [&](int sweep) {
     std::vector<int32_t> work;
     int max_nb = 0, max_ell = 0;
@@ -165,6 +169,7 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     }
     if (work.empty()) return false;
     const int slot = sweep & 1;
+    slot_rounds[size_t(slot)] = int64_t(work.size() / 2) * (max_nb - 1);
     const int* skip = sweep > 0 ? d_cnt[slot ^ 1] : nullptr;
     kern::fill_i32(ctx->stream(), d_cnt[slot], ns, 0);
     auto* d_work = ctx->upload_vec(work);
@@ -180,19 +185,24 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   };
   if (!enqueue(0)) return 0;
   int queued = 1, sweeps = 0;
+  bool speculate = true;
   for (int sweep = 0; sweep < queued; ++sweep) {
-    if (queued < max_sweeps && enqueue(queued)) ++queued;
+    if (speculate && queued < max_sweeps && enqueue(queued)) ++queued;
     ctx->wait_event(ctx->read_event(sweep & 1));
     const int32_t* h_cnt = ctx->read_slot(sweep & 1);
     sweeps = sweep + 1;
     if (sweep == 0 && first_rot)
       for (size_t q = 0; q < which.size(); ++q) (*first_rot)[q] = h_cnt[which[q]];
     bool any = false;
+    int64_t rotated = 0;
     for (int i = 0; i < ns; ++i) {
+      rotated += h_cnt[i];
       if (active[i] && h_cnt[i] == 0) active[i] = 0;
       any |= bool(active[i]);
     }
     if (!any) break;
+    speculate = 4 * rotated >= slot_rounds[size_t(sweep & 1)];
+    if (queued == sweep + 1 && queued < max_sweeps && enqueue(queued)) ++queued;
   }
   return sweeps;
 }
