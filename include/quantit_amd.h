@@ -46,6 +46,11 @@ int qtt_context_destroy(qtt_context* ctx);
 int qtt_context_set_precision(qtt_context* ctx, int precision);
 /* 1: one GEMM launch per block pair (reference-shaped baseline) */
 int qtt_context_set_per_block(qtt_context* ctx, int enable);
+/* 1 (default): independent work of one call (the narrow-sector solves of a
+   block SVD beside the wide ones) may run on a second stream; 0: one stream.
+   No counterpart in the reference.  Processes that drive several contexts at
+   once (concurrent solves) set 0: their streams already share the device. */
+int qtt_context_set_overlap(qtt_context* ctx, int enable);
 int qtt_context_set_profiling(qtt_context* ctx, int enable);
 int qtt_context_sync(qtt_context* ctx);
 int qtt_context_memory(qtt_context* ctx, size_t* used, size_t* peak, size_t* capacity);

@@ -76,6 +76,7 @@ SIGNATURES = {
     "qtt_context_destroy": (C.c_int, [P]),
     "qtt_context_set_precision": (C.c_int, [P, C.c_int]),
     "qtt_context_set_per_block": (C.c_int, [P, C.c_int]),
+    "qtt_context_set_overlap": (C.c_int, [P, C.c_int]),
     "qtt_context_set_profiling": (C.c_int, [P, C.c_int]),
     "qtt_context_sync": (C.c_int, [P]),
     "qtt_context_memory": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
@@ -151,6 +152,11 @@ class Context:
 
     def set_per_block(self, enable):
         _check(library().qtt_context_set_per_block(self.h, int(bool(enable))))
+
+    def set_overlap(self, enable):
+        """Allow (default) or forbid a second stream for independent work of
+        one call; processes running several contexts at once forbid it."""
+        _check(library().qtt_context_set_overlap(self.h, int(bool(enable))))
 
     def set_profiling(self, enable):
         _check(library().qtt_context_set_profiling(self.h, int(bool(enable))))

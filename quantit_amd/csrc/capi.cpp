@@ -79,6 +79,13 @@ int qtt_context_set_per_block(qtt_context* c, int enable) {
   });
 }
 
+int qtt_context_set_overlap(qtt_context* c, int enable) {
+  return guard([&] {
+    need(c, "ctx");
+    c->ctx.overlap = enable != 0;
+  });
+}
+
 int qtt_context_set_profiling(qtt_context* c, int enable) {
   return guard([&] {
     need(c, "ctx");

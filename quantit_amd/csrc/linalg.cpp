@@ -616,7 +616,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
       // sectors' Cholesky and rounds (which leave most CUs free); the main
       // stream joins before the rotation.  Profiling keeps one stream so that
       // the event timers stay per kernel.
-      const bool overlap = !act_s.empty() && !act_l.empty() && !ctx->profiling;
+      const bool overlap = !act_s.empty() && !act_l.empty() && !ctx->profiling && ctx->overlap;
       if (!act_s.empty()) {
         std::vector<kern::SmallJacobi> jd;
         for (int q : act_s) {

@@ -74,7 +74,6 @@ void Arena::release(size_t off) {
 Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(device) {
   QTT_HIP(hipSetDevice(device));
   QTT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  QTT_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   QTT_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
   QTT_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
   arena_ = std::make_unique<Arena>(arena_bytes);
@@ -112,7 +111,7 @@ void* Context::upload(const void* data, size_t bytes) {
     // wrap: everything previously staged must have been consumed, by the
     // work of both streams
     QTT_HIP(hipStreamSynchronize(stream_));
-    QTT_HIP(hipStreamSynchronize(side_));
+    if (side_) QTT_HIP(hipStreamSynchronize(side_));
     stage_pos_ = 0;
   }
   char* h = host_stage_ + stage_pos_;
@@ -126,6 +125,7 @@ void* Context::upload(const void* data, size_t bytes) {
 }
 
 void Context::fork() {
+  if (!side_) QTT_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
   QTT_HIP(hipEventRecord(fork_ev_, stream_));
   QTT_HIP(hipStreamWaitEvent(side_, fork_ev_, 0));
 }
@@ -140,16 +140,7 @@ void Context::sync() {
   wait_event(sync_ev_);
 }
 
-void Context::wait_event(hipEvent_t e) {
-  const auto t0 = std::chrono::steady_clock::now();
-  for (;;) {
-    const hipError_t r = hipEventQuery(e);
-    if (r == hipSuccess) return;
-    if (r != hipErrorNotReady) QTT_HIP(r);
-    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
-  }
-  QTT_HIP(hipEventSynchronize(e));
-}
+void Context::wait_event(hipEvent_t e) { QTT_HIP(hipEventSynchronize(e)); }
 
 void Context::read_small(void* host_dst, const void* dev_src, size_t bytes) {
   constexpr size_t kOff = 256;

@@ -70,10 +70,11 @@ class Context {
 
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
-  // A second stream for independent work beside the main one: fork() makes
-  // it wait for everything enqueued on the main stream so far, join() makes
-  // the main stream wait for everything enqueued on it.  Both are device-side
-  // waits (no host synchronization).
+  // A second stream for independent work beside the main one, created on
+  // the first fork() (a context that never forks keeps one stream, and so
+  // one hardware queue): fork() makes it wait for everything enqueued on the
+  // main stream so far, join() makes the main stream wait for everything
+  // enqueued on it.  Both are device-side waits (no host synchronization).
   hipStream_t side_stream() const { return side_; }
   void fork();
   void join();
@@ -107,8 +108,8 @@ class Context {
     return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
   }
   void sync();
-  // Waits for an event recorded on this stream: polls it for up to 2 ms (a
-  // short wait then ends without a sleep and wake-up), then blocks.
+  // Waits for an event recorded on this stream.  (Polling the event instead
+  // made concurrent contexts of one process slower.)
   void wait_event(hipEvent_t e);
 
   // Pinned host slots for small device-to-host reads that the host waits on
@@ -129,6 +130,11 @@ class Context {
   Precision precision = Precision::FP64;
   // Reference-shaped baseline: one GEMM launch per block pair (mm, addmm_).
   bool per_block_launch = false;
+  // Independent work of one call may run on the side stream.  Processes that
+  // run several contexts at once turn it off: their streams already share the
+  // device, and a side stream takes a hardware queue that another context's
+  // stream then waits behind.
+  bool overlap = true;
   // Per-kernel accumulated time is recorded when profiling is on.
   bool profiling = false;
   // Timer name of the contraction GEMMs (callers label their call sites;

@@ -339,19 +339,21 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     if (used_a[k]) bytes += 8.0 * double(a.blocks[k].numel);
   for (size_t k = 0; k < b.blocks.size(); ++k)
     if (used_b[k]) bytes += 8.0 * double(b.blocks[k].numel);
-  // Narrow blocks (few columns, short summed K: an MFMA tile would be mostly
-  // padding) go to the row kernel (FP64 mode); the others to MFMA tiles.
+  // A contraction whose blocks are all narrow (few columns, short summed K:
+  // an MFMA tile would be mostly padding) runs on the row kernel (FP64
+  // mode); otherwise every block takes MFMA tiles, so a contraction is one
+  // launch either way.
   const char* rows_env = std::getenv("QTT_GEMM_ROWS");  // "0": MFMA tiles for every block
-  const bool rows_ok = ctx->precision == Precision::FP64 && !ctx->per_block_launch &&
-                       !(rows_env && std::atoi(rows_env) == 0);
-  std::vector<char> narrow(outs.size(), 0);
+  bool all_narrow = ctx->precision == Precision::FP64 && !ctx->per_block_launch &&
+                    !(rows_env && std::atoi(rows_env) == 0);
+  for (size_t o = 0; o < outs.size() && all_narrow; ++o)
+    all_narrow = gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK;
+  std::vector<char> narrow(outs.size(), char(all_narrow));
   int64_t mfma_tiles = 0;
-  for (size_t o = 0; o < outs.size(); ++o) {
-    narrow[o] = rows_ok && gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK;
-    if (!narrow[o])
+  if (!all_narrow)
+    for (size_t o = 0; o < outs.size(); ++o)
       mfma_tiles += int64_t((gouts[o].M + kern::kGemmTile - 1) / kern::kGemmTile) *
                     ((gouts[o].N + kern::kGemmTile - 1) / kern::kGemmTile);
-  }
   // Split-K when the MFMA tiles would leave most CUs idle (long-K
   // contractions with few output tiles, such as the right environment of
   // the matvec): the K extent of a block, its pairs in order, is cut into

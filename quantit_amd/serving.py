@@ -105,10 +105,17 @@ class Server:
         self.device = device
         self.solve_kw = solve_kw
         self.solve_fn = solve_fn or solve
-        self.context_fn = context_fn or (lambda nbytes: Context(self.device, arena_bytes=nbytes))
+        self.context_fn = context_fn or self._context
         self.free = hbm_budget
         self.min_free = hbm_budget
         self.cv = threading.Condition()
+
+    def _context(self, nbytes):
+        ctx = Context(self.device, arena_bytes=nbytes)
+        # several solves at once: one stream each (a second stream per solve
+        # would share hardware queues with the other solves' streams)
+        ctx.set_overlap(self.n_slots == 1)
+        return ctx
 
     def run(self, trace, time_scale=1.0, progress=None):
         """Replays the trace (arrival times scaled by time_scale) and returns

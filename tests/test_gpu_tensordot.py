@@ -61,9 +61,9 @@ def test_tensordot_matches_oracle(ctx, case, groups):
 
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_tensordot_narrow_rows_equal_mfma_tiles(ctx, case, monkeypatch):
-    """Narrow output blocks (few columns, short summed K) take the row kernel;
-    the same contraction with every block on MFMA tiles agrees to 1e-13, and
-    the row kernel is seen to run."""
+    """A contraction whose output blocks are all narrow (few columns, short
+    summed K) takes the row kernel; the same contraction on MFMA tiles agrees
+    to 1e-13, and the row kernel is seen to run where every block is narrow."""
     (ra, rb), da, db = CASES[case]
     rng = np.random.default_rng(300 + case)
     A, B = _pair(rng, G1, ra, rb, da, db, nsec=3 if ra + rb > 8 else 4)
