@@ -193,6 +193,8 @@ struct CholPanel {
 };
 // One panel of nb columns starting at j0; LDS kmax * nb doubles.
 void pchol_panel(hipStream_t s, double* base, const void* descs, int n, int j0, int nb, int kmax);
+// state of every factorisation: [0] = k (running), [1 .. k] = 0 (none pivoted)
+void pchol_init(hipStream_t s, double* base, const void* descs, int n);
 constexpr int kPcholExtra = 2;  // panel LDS per index besides the columns: running diagonal, flag
 size_t pchol_lds_limit();
 struct GramVec {

@@ -1533,11 +1533,22 @@ __global__ __launch_bounds__(1024) void k_truncate(double* __restrict__ base, co
     int64_t last = total - 1;
     double x = fabs(v[last]);
     double trunc = sq ? x * x : pow(x, pw);
-    while (last >= min_size && last > 0) {
-      if (trunc > toln && last < max_size) break;
-      --last;
-      x = fabs(v[last]);
-      trunc += sq ? x * x : pow(x, pw);
+    // the same steps in the same order, the next eight values read ahead
+    // (a dependent read per step would dominate the scan)
+    for (bool done = false; !done;) {
+      double buf[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) buf[j] = last - 1 - j >= 0 ? v[last - 1 - j] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (!(last >= min_size && last > 0) || (trunc > toln && last < max_size)) {
+          done = true;
+          break;
+        }
+        --last;
+        x = fabs(buf[j]);
+        trunc += sq ? x * x : pow(x, pw);
+      }
     }
     const double sl = v[last];
     thr_sh = sl - 2.0 * sl * DBL_EPSILON;

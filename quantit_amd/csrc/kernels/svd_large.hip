@@ -247,6 +247,17 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
   }
 }
 
+__global__ __launch_bounds__(256) void k_pchol_init(double* __restrict__ base, const CholPanel* __restrict__ ps) {
+  const CholPanel P = ps[blockIdx.x];
+  int* state = reinterpret_cast<int*>(base + P.state_off);
+  for (int i = threadIdx.x; i <= P.k; i += blockDim.x) state[i] = i == 0 ? P.k : 0;
+}
+
+void pchol_init(hipStream_t st, double* base, const void* descs, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pchol_init, dim3(n), dim3(256), 0, st, base, static_cast<const CholPanel*>(descs));
+}
+
 void pchol_panel(hipStream_t st, double* base, const void* descs, int n, int j0, int nb, int kmax) {
   if (n <= 0) return;
   const size_t lds = size_t(kmax) * (nb + kPcholExtra) * sizeof(double);

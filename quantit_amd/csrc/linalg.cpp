@@ -550,11 +550,6 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         KernelTimer kt(*ctx, "svd_init");
         auto [dz, dzt] = ctx->upload_pack(z, zt);
         kern::copy2d(ctx->stream(), ab, dz, dzt, int(zt.size()));
-        for (int q : qs) {
-          int32_t* st = reinterpret_cast<int32_t*>(ab + gb + sto[q]);
-          kern::fill_i32(ctx->stream(), st, B.secs[wide[q]].k + 1, 0);
-          kern::fill_i32(ctx->stream(), st, 1, B.secs[wide[q]].k);
-        }
       }
       int kact = 0;
       for (int q : qs) kact = std::max(kact, B.secs[wide[q]].k);
@@ -578,6 +573,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         prs.push_back(gp);
       }
       const auto* d_pd = ctx->upload_vec(pd);
+      kern::pchol_init(ctx->stream(), ab, d_pd, int(pd.size()));
       const GemmLaunch upd = prepare_gemm(ctx, outs, prs);
       for (int j0 = 0; j0 < kact; j0 += pnb) {
         {
