@@ -146,9 +146,14 @@ void svd_init_w(hipStream_t s, double* base, const SvdSector* secs, int nsec);
 // cross_after0: rounds after the first of a sweep rotate only the pairs
 // across the two blocks (the first round takes every pair of its blocks).
 // skip (may be null): the previous sweep's counters; a sector whose entry is
-// 0 has converged and its work-groups return at once.
-void svd_round(hipStream_t s, double* base, const SvdSector* secs, const int32_t* work, int nwork, int round,
-               int* counters, const int* skip, double tol, int inner_sweeps, int max_ell, int cross_after0);
+// 0 has converged and its work-groups return at once.  A work item carries
+// the sector fields a round needs, so a work-group starts with one load.
+struct RoundWork {
+  int64_t y_off, w_off;
+  int32_t ell, nb, flag_idx, pair;
+};
+void svd_round(hipStream_t s, double* base, const RoundWork* work, int nwork, int round, int* counters,
+               const int* skip, double tol, int inner_sweeps, int max_ell, int cross_after0);
 // norms + ranks (one work-group per sector), tiled transposed scatter of the
 // (sector, part, row0, col0) tasks into U / V, orthonormal completion.
 void svd_finalize(hipStream_t s, double* base, const SvdSector* secs, const SvdSector* host_secs, int nsec,

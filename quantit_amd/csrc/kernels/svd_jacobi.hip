@@ -138,8 +138,8 @@ __device__ void apply_rotation(double* __restrict__ M, int64_t ld, int rows, con
 // computed from there and written back once; otherwise 128-row chunks are
 // staged twice (Gram, then rotation).
 template <bool RESIDENT>
-__global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __restrict__ base, const SvdSector* __restrict__ secs,
-                                                   const int32_t* __restrict__ work, int round,
+__global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __restrict__ base,
+                                                   const RoundWork* __restrict__ work, int round,
                                                    int* __restrict__ counters, const int* __restrict__ skip,
                                                    double tol, int inner_sweeps, int pitch, int cross_after0) {
   extern __shared__ double dyn[];
@@ -149,8 +149,8 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
   __shared__ int flag, nrot;
   double (*Ys)[YP] = reinterpret_cast<double (*)[YP]>(dyn);  // chunk staging (aliases the resident block)
 
-  const SvdSector s = secs[work[2 * blockIdx.x]];
-  const int pi = work[2 * blockIdx.x + 1];
+  const RoundWork s = work[blockIdx.x];
+  const int pi = s.pair;
   const int nb = s.nb;
   if (round >= nb - 1) return;
   // a sector whose previous sweep rotated nothing has converged
@@ -431,18 +431,18 @@ int svd_round_resident_rows() {
   return rows;
 }
 
-void svd_round(hipStream_t st, double* base, const SvdSector* secs, const int32_t* work, int nwork, int round,
-               int* counters, const int* skip, double tol, int inner_sweeps, int max_ell, int cross_after0) {
+void svd_round(hipStream_t st, double* base, const RoundWork* work, int nwork, int round, int* counters,
+               const int* skip, double tol, int inner_sweeps, int max_ell, int cross_after0) {
   if (nwork <= 0) return;
   const size_t staging = size_t(NB2) * YP * sizeof(double);
   if (max_ell <= svd_round_resident_rows()) {
     const int pitch = max_ell + 1;
     const size_t lds = std::max(size_t(NB2) * pitch * sizeof(double), staging);
-    hipLaunchKernelGGL(k_svd_round<true>, dim3(nwork), dim3(1024), lds, st, base, secs, work, round, counters, skip,
-                       tol, inner_sweeps, pitch, cross_after0);
+    hipLaunchKernelGGL(k_svd_round<true>, dim3(nwork), dim3(1024), lds, st, base, work, round, counters, skip, tol,
+                       inner_sweeps, pitch, cross_after0);
   } else {
-    hipLaunchKernelGGL(k_svd_round<false>, dim3(nwork), dim3(256), staging, st, base, secs, work, round, counters,
-                       skip, tol, inner_sweeps, 0, cross_after0);
+    hipLaunchKernelGGL(k_svd_round<false>, dim3(nwork), dim3(256), staging, st, base, work, round, counters, skip,
+                       tol, inner_sweeps, 0, cross_after0);
   }
 }
 

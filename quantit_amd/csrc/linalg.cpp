@@ -142,7 +142,6 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
                   double tol, int inner, int max_sweeps, std::vector<int>* first_rot) {
   const int ns = int(secs.size());
   if (which.empty() || max_sweeps <= 0) return 0;
-  auto* d_secs = ctx->upload_vec(secs);
   BufPtr cnt = ctx->alloc(size_t(2 * ns) * sizeof(int32_t));
   int32_t* d_cnt[2] = {reinterpret_cast<int32_t*>(cnt->cptr()), reinterpret_cast<int32_t*>(cnt->cptr()) + ns};
   ctx->reserve_read(size_t(ns));
@@ -155,27 +154,24 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   // enqueues sweep `sweep` over the active sectors and the read of its counts
   std::vector<int64_t> slot_rounds(2, 0);  // pair-rounds enqueued per slot
   auto enqueue = [&](int sweep) {
-    std::vector<int32_t> work;
+    std::vector<kern::RoundWork> work;
     int max_nb = 0, max_ell = 0;
     for (int i = 0; i < ns; ++i) {
       if (!active[i]) continue;
-      const int nb = secs[i].nb;
-      max_nb = std::max(max_nb, nb);
-      max_ell = std::max(max_ell, secs[i].ell);
-      for (int pi = 0; pi < nb / 2; ++pi) {
-        work.push_back(i);
-        work.push_back(pi);
-      }
+      const auto& sc = secs[i];
+      max_nb = std::max(max_nb, sc.nb);
+      max_ell = std::max(max_ell, sc.ell);
+      for (int pi = 0; pi < sc.nb / 2; ++pi) work.push_back({sc.y_off, sc.w_off, sc.ell, sc.nb, sc.flag_idx, pi});
     }
     if (work.empty()) return false;
     const int slot = sweep & 1;
-    slot_rounds[size_t(slot)] = int64_t(work.size() / 2) * (max_nb - 1);
+    slot_rounds[size_t(slot)] = int64_t(work.size()) * (max_nb - 1);
     const int* skip = sweep > 0 ? d_cnt[slot ^ 1] : nullptr;
     kern::fill_i32(ctx->stream(), d_cnt[slot], ns, 0);
     auto* d_work = ctx->upload_vec(work);
     for (int r = 0; r < max_nb - 1; ++r) {
       KernelTimer kt(*ctx, "svd_round");
-      kern::svd_round(ctx->stream(), ab, d_secs, d_work, int(work.size() / 2), r, d_cnt[slot], skip, tol, inner,
+      kern::svd_round(ctx->stream(), ab, d_work, int(work.size()), r, d_cnt[slot], skip, tol, inner,
                       max_ell, cross == 1 || (cross == 2 && sweep > 0));
     }
     QTT_HIP(hipMemcpyAsync(ctx->read_slot(slot), d_cnt[slot], size_t(ns) * sizeof(int32_t), hipMemcpyDeviceToHost,
