@@ -368,7 +368,7 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   std::vector<kern::ChunkTask> sum_tasks;
   BufPtr partials;
   {
-    constexpr int64_t kTargetTiles = 1024, kSplitMinK = 128;
+    constexpr int64_t kTargetTiles = 2048, kSplitMinK = 64;
     constexpr int64_t kMaxSplit = 8;
     const char* split_env = std::getenv("QTT_GEMM_SPLIT_K");  // "0": never split
     const bool split_ok = !ctx->per_block_launch && !(split_env && std::atoi(split_env) == 0);
