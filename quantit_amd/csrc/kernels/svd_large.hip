@@ -229,9 +229,12 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
         v = ljj;
         used[p] = 1;
       } else if (!used[i]) {
-        v = G[int64_t(p) * k + i];
-        for (int t = 0; t < jj; ++t) v -= col[t * k + i] * col[t * k + p];
-        v *= isq;
+        // the load of G's column p is independent of the dot product over the
+        // panel's earlier columns, so its latency overlaps that sum
+        const double gpi = G[int64_t(p) * k + i];
+        double acc = 0.0;
+        for (int t = 0; t < jj; ++t) acc = fma(col[t * k + i], col[t * k + p], acc);
+        v = (gpi - acc) * isq;
         dg[i] -= v * v;
       }
       col[jj * k + i] = v;
