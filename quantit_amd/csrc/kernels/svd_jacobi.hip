@@ -207,7 +207,21 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
     const double* ya = Yl + (ti * 16 + cl) * pitch + kq;
     const double* yb = Yl + (tj * 16 + cl) * pitch + kq;
     v4d g1 = v4d{0.0, 0.0, 0.0, 0.0};
-    if (gw) {
+    if (gw && ti == tj) {  // diagonal tile: both operands are the same columns
+      int kk = 4 * part;
+      for (; kk + 4 * kParts < ell; kk += 8 * kParts) {
+        const bool in1 = kk + 4 * kParts + kq < ell;
+        const double a0 = ya[kk];
+        const double a1 = in1 ? ya[kk + 4 * kParts] : 0.0;
+        g = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, a0, g, 0, 0, 0);
+        g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, a1, g1, 0, 0, 0);
+      }
+      for (; kk < ell; kk += 4 * kParts) {
+        const double xa = kk + kq < ell ? ya[kk] : 0.0;
+        g = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, xa, g, 0, 0, 0);
+      }
+      g = g + g1;
+    } else if (gw) {
       int kk = 4 * part;
       for (; kk + 4 * kParts < ell; kk += 8 * kParts) {
         const bool in1 = kk + 4 * kParts + kq < ell;
