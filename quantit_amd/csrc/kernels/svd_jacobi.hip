@@ -429,28 +429,32 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
 
 // Rows of a block pair that stay resident in LDS (dynamic size at most the
 // device's per-block LDS less the static arrays).
-int svd_round_resident_rows() {
-  static int rows = -1;
-  if (rows < 0) {
+int svd_round_dyn_max() {
+  static int dyn_max = -1;
+  if (dyn_max < 0) {
     int dev = 0, v = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    const int dyn_max = v - int(2 * NB2 * GP * sizeof(double)) - 1024;
-    rows = dyn_max / int(NB2 * sizeof(double)) - 1;
+    dyn_max = v - int(2 * NB2 * GP * sizeof(double)) - 1024;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_svd_round<true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_svd_round<false>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
   }
-  return rows;
+  return dyn_max;
 }
+
+// LDS pitch (doubles) of a resident column: at least ell and 2 (mod 32), so
+// that a 32-lane phase of the Gram's reads (16 columns x 2 rows) meets each
+// of the 64 banks once.
+int resident_pitch(int ell) { return (ell + 29) / 32 * 32 + 2; }
 
 void svd_round(hipStream_t st, double* base, const RoundWork* work, int nwork, int round, int* counters,
                const int* skip, double tol, int inner_sweeps, int max_ell, int cross_after0) {
   if (nwork <= 0) return;
   const size_t staging = size_t(NB2) * YP * sizeof(double);
-  if (max_ell <= svd_round_resident_rows()) {
-    const int pitch = max_ell + 1;
+  const int pitch = resident_pitch(max_ell);
+  if (int64_t(pitch) * NB2 * int64_t(sizeof(double)) <= svd_round_dyn_max()) {
     const size_t lds = std::max(size_t(NB2) * pitch * sizeof(double), staging);
     hipLaunchKernelGGL(k_svd_round<true>, dim3(nwork), dim3(1024), lds, st, base, work, round, counters, skip, tol,
                        inner_sweeps, pitch, cross_after0);
