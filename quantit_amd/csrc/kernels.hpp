@@ -41,18 +41,14 @@ void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, co
 void gemm_split(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
                 const GemmPair* pairs, double alpha, double beta);
 // Output blocks with at most kRowsMaxN columns and a short summed K: one
-// lane per output element, the pairs in order, k ascending (one FP64 FMA
-// chain per element).  A task is kRowsPerTask / npad consecutive rows of one
-// output block, npad = N rounded up to a power of two (rows_per_task()).
+// thread per output row (B staged in LDS), the pairs in order, k ascending
+// (one FP64 FMA chain per element).  A task is kRowsPerTask consecutive rows
+// of one output block.
 constexpr int kRowsMaxN = 16, kRowsMaxK = 128, kRowsPerTask = 256;
 struct RowTask {
   int32_t out, row0;
 };
-inline int rows_per_task(int n) {
-  int npad = 1;
-  while (npad < n) npad <<= 1;
-  return kRowsPerTask / npad;
-}
+inline int rows_per_task(int) { return kRowsPerTask; }
 void gemm_rows(hipStream_t s, double* base, const RowTask* tasks, int ntasks, const GemmOut* outs,
                const GemmPair* pairs, double alpha, double beta);
 

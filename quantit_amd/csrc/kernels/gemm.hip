@@ -304,33 +304,57 @@ __global__ __launch_bounds__(256) void k_gemm_split(double* __restrict__ base, c
 }
 
 // Narrow outputs (N <= kRowsMaxN, summed K <= kRowsMaxK): an MFMA tile
-// would be mostly padding there.  One lane per output element: a row takes
-// npad = N rounded up to a power of two consecutive lanes, so the lanes of a
-// row share each load of A (one address) and read B along n.  Every element
-// is one FP64 FMA chain over the pairs in order, k ascending.
+// would be mostly padding there.  One thread per output row: B of every
+// pair (at most kRowsMaxK x kRowsMaxN) is staged in LDS once per work-group,
+// each thread streams its row of A (lanes read consecutive rows when A's row
+// stride is 1) and keeps the N accumulators in registers.  Every element is
+// one FP64 FMA chain over the pairs in order, k ascending.
 __global__ __launch_bounds__(kRowsPerTask) void k_gemm_rows(double* __restrict__ base,
                                                              const RowTask* __restrict__ tasks,
                                                              const GemmOut* __restrict__ outs,
                                                              const GemmPair* __restrict__ pairs, double alpha,
                                                              double beta) {
+  __shared__ double Bs[kRowsMaxK * kRowsMaxN];
   const RowTask t = tasks[blockIdx.x];
   const GemmOut o = outs[t.out];
-  int npad = 1;
-  while (npad < o.N) npad <<= 1;
-  const int m = t.row0 + int(threadIdx.x) / npad, n = int(threadIdx.x) % npad;
-  if (m >= o.M || n >= o.N) return;
-  double acc = 0.0;
+  int kb = 0;
+  for (int p = o.pair_begin; p < o.pair_end; ++p) {
+    const GemmPair pr = pairs[p];
+    const double* B = base + pr.b_off;
+    for (int e = int(threadIdx.x); e < pr.K * o.N; e += blockDim.x) {
+      const int k = e / o.N, n = e - k * o.N;
+      Bs[(kb + k) * kRowsMaxN + n] = B[int64_t(k) * pr.b_rs + int64_t(n) * pr.b_cs];
+    }
+    kb += pr.K;
+  }
+  __syncthreads();
+  const int m = t.row0 + int(threadIdx.x);
+  if (m >= o.M) return;
+  double acc[kRowsMaxN];
+#pragma unroll
+  for (int n = 0; n < kRowsMaxN; ++n) acc[n] = 0.0;
+  kb = 0;
   for (int p = o.pair_begin; p < o.pair_end; ++p) {
     const GemmPair pr = pairs[p];
     const double* A = base + pr.a_off + int64_t(m) * pr.a_rs;
-    const double* B = base + pr.b_off + int64_t(n) * pr.b_cs;
 #pragma unroll 4
-    for (int k = 0; k < pr.K; ++k) acc = fma(A[int64_t(k) * pr.a_cs], B[int64_t(k) * pr.b_rs], acc);
+    for (int k = 0; k < pr.K; ++k) {
+      const double a = A[int64_t(k) * pr.a_cs];
+      const double* bk = Bs + (kb + k) * kRowsMaxN;
+#pragma unroll
+      for (int n = 0; n < kRowsMaxN; ++n)
+        if (n < o.N) acc[n] = fma(a, bk[n], acc[n]);
+    }
+    kb += pr.K;
   }
-  double* C = base + o.c_off + int64_t(m) * o.ldc + n;
-  double v = alpha * acc;
-  if (beta != 0.0) v += beta * (*C);
-  *C = v;
+  double* C = base + o.c_off + int64_t(m) * o.ldc;
+#pragma unroll
+  for (int n = 0; n < kRowsMaxN; ++n)
+    if (n < o.N) {
+      double v = alpha * acc[n];
+      if (beta != 0.0) v += beta * C[n];
+      C[n] = v;
+    }
 }
 
 void gemm_rows(hipStream_t s, double* base, const RowTask* tasks, int ntasks, const GemmOut* outs,
