@@ -299,7 +299,6 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   std::vector<kern::GemmPair> gpairs;
   gpairs.reserve(all_pairs.size());
   std::vector<int64_t> out_work(outs.size());
-  int64_t ntiles = 0;
   double flops = 0, bytes = 0;
   for (size_t o = 0; o < outs.size(); ++o) {
     const Block& cb_ = out.blocks[o];
@@ -332,7 +331,6 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     }
     gouts[o].pair_end = int32_t(gpairs.size());
     out_work[o] = ksum;
-    ntiles += ((M + kern::kGemmTile - 1) / kern::kGemmTile) * ((N + kern::kGemmTile - 1) / kern::kGemmTile);
     bytes += 8.0 * double(cb_.numel);
   }
   for (size_t k = 0; k < a.blocks.size(); ++k)
