@@ -148,9 +148,11 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   std::vector<char> active(ns, 0);
   for (int i : which) active[i] = 1;
   if (first_rot) first_rot->assign(which.size(), 0);
-  // From the second sweep on, rounds after the first of a sweep rotate only
-  // the pairs across their two blocks (0: never, 1: from the first sweep).
-  const int cross = env_int("QTT_SVD_CROSS_INNER", 2);
+  // Rounds after the first of a sweep rotate only the pairs across their two
+  // blocks (the first round holds every block once and takes all its pairs,
+  // so a sweep still meets every column pair).  QTT_SVD_CROSS_INNER: 1 (the
+  // default) from the first sweep on, 2 from the second, 0 never.
+  const int cross = env_int("QTT_SVD_CROSS_INNER", 1);
   // enqueues sweep `sweep` over the active sectors and the read of its counts
   std::vector<int64_t> slot_rounds(2, 0);  // pair-rounds enqueued per slot
   auto enqueue = [&](int sweep) {
