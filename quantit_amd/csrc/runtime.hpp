@@ -108,8 +108,9 @@ class Context {
     return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
   }
   void sync();
-  // Waits for an event recorded on this stream.  (Polling the event instead
-  // made concurrent contexts of one process slower.)
+  // Waits for an event recorded on this stream (a blocking wait: the host
+  // thread sleeps instead of polling, which matters when several solve
+  // threads wait at once).
   void wait_event(hipEvent_t e);
 
   // Pinned host slots for small device-to-host reads that the host waits on
