@@ -56,6 +56,28 @@ int qtt_context_sync(qtt_context* ctx);
 int qtt_context_memory(qtt_context* ctx, size_t* used, size_t* peak, size_t* capacity);
 /* Writes "name ms launches\n" lines of the accumulated per-kernel times. */
 int qtt_context_kernel_times(qtt_context* ctx, char* buf, size_t buflen, int reset);
+/* Block SVD/eigh health since the context was created: calls, the largest
+   number of sweeps (Gram iterations + Jacobi sweeps) one call needed, and
+   sectors whose Jacobi sweeps reached the sweep cap while still rotating
+   (their singular values are then less accurate than the tolerance).  No
+   counterpart in the reference, whose LAPACK svd raises on non-convergence. */
+int qtt_context_svd_stats(qtt_context* ctx, int64_t* calls, int64_t* max_sweeps, int64_t* unconverged);
+/* Block SVD model FLOPs (sum over sectors of 14 m n^2 + 8 n^3, m >= n) since
+   the context was created: all sectors, and the sectors whose rotations ran
+   as multi-work-group block-Jacobi rounds (the wide ones). */
+int qtt_context_svd_flops(qtt_context* ctx, double* all, double* wide);
+/* Eigensolver of the DMRG two-site update.  max_krylov = 0 (default): the
+   reference's one-step Lanczos with its closed-form 2x2 solve
+   (details::one_step_lanczos + eig2x2Mat, sources/dmrg.cpp:543-608).
+   max_krylov in [2, 64]: k-step Lanczos with full FP64 reorthogonalisation
+   and an FP64 tridiagonal eigensolve on the device, stopping when the
+   residual |H theta - E theta| <= tol |E|, at an invariant subspace or at
+   max_krylov vectors (2 reproduces the one-step update).  No counterpart in
+   the reference; it lets a solve converge to the energy tolerance instead
+   of improving by one Lanczos step per update. */
+int qtt_context_set_lanczos(qtt_context* ctx, int max_krylov, double tol);
+/* k-step mode: two-site updates run and Krylov vectors built so far. */
+int qtt_context_lanczos_stats(qtt_context* ctx, int64_t* updates, int64_t* vectors);
 
 /* ---- btensor ------------------------------------------------------------
  * Replaces btensor(index_list sections_by_dim, any_quantity_vector c_vals,
@@ -95,7 +117,6 @@ int qtt_tensordot(const qtt_btensor* a, const qtt_btensor* b, int ndims, const i
                   const int64_t* dims_b, qtt_btensor** out);
 /* Host-counted algorithmic FLOPs (sum of 2 M N K) of the last tensordot on this thread. */
 double qtt_last_tensordot_flops(void);
-
 /* ---- elementwise ------------------------------------------------------------ */
 /* broadcast mul (btensor.h:547, sources/btensor.cpp:1750) */
 int qtt_mul(const qtt_btensor* x, const qtt_btensor* y, qtt_btensor** out);
@@ -132,6 +153,27 @@ typedef struct qtt_dmrg_options {
 int qtt_dmrg(int L, qtt_btensor* const* mpo, qtt_btensor** mps, const qtt_dmrg_options* options, double* energy,
              int max_log, double* sweep_energies, double* sweep_ms, int64_t* max_bond, int* n_sweeps,
              double* contract_flops, double* svd_flops);
+/* The same with the state's orthogonality centre (bMPS::orthogonality_center)
+ * at oc instead of 0: sweeps start toward the far end, a centre on the last
+ * site starts one site to the left, and the centre is moved back to oc at
+ * the end (details::dmrg_impl, sources/dmrg.cpp:224-236, :257-266). */
+int qtt_dmrg_from_center(int L, qtt_btensor* const* mpo, qtt_btensor** mps, int oc, const qtt_dmrg_options* options,
+                         double* energy, int max_log, double* sweep_energies, double* sweep_ms, int64_t* max_bond,
+                         int* n_sweeps, double* contract_flops, double* svd_flops);
+/* details::hamil2site_times_state (sources/dmrg.cpp:520-531): the two-site
+ * matvec L.theta.W12.R (three tensordots). */
+int qtt_hamil2site_times_state(const qtt_btensor* theta, const qtt_btensor* W12, const qtt_btensor* L,
+                               const qtt_btensor* R, qtt_btensor** out);
+/* details::one_step_lanczos (sources/dmrg.cpp:584-608): Hpsi receives the
+ * normalised Lanczos vector (H theta - a0 theta)/b (undivided when |b| <
+ * 1e-15); a0 = <theta|H|theta>, a1 = <Hpsi|H|Hpsi>, b the residual norm. */
+int qtt_one_step_lanczos(const qtt_btensor* theta, const qtt_btensor* W12, const qtt_btensor* L,
+                         const qtt_btensor* R, qtt_btensor** Hpsi, double* a0, double* a1, double* b);
+/* details::eig2x2Mat (sources/dmrg.cpp:543-572) evaluated on the context's
+ * device: lowest eigenvalue E of [[a0, b], [b, a1]] and its eigenvector
+ * (o, n).  A NaN gives QTT_LOGIC_ERROR, as the reference's std::logic_error. */
+int qtt_eig2x2(qtt_context* ctx, double a0, double a1, double b, double* E, double* o, double* n);
+
 /* bMPS::move_oc (sources/MPT.cpp:74-111): moves the orthogonality centre of
  * the L tensors mps[] from oc to target with untruncated block SVDs; the
  * handles are replaced in place. */

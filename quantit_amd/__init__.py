@@ -20,11 +20,15 @@ LIB_PATH = _PKG / "libquantit_amd.so"
 PRECISION_FP64 = 0
 PRECISION_SPLIT_FP32 = 1
 
+# The Python types pybind11 gives the reference's exceptions in its own
+# binding (python_binding/*.cpp): invalid_argument and domain_error become
+# ValueError, out_of_range IndexError, logic_error (no translator of its
+# own) RuntimeError, and torch's translator maps c10::Error to RuntimeError.
 _STATUS_EXC = {
     1: ValueError,        # std::invalid_argument
     2: IndexError,        # std::out_of_range
-    3: ArithmeticError,   # std::domain_error
-    4: ArithmeticError,   # std::logic_error (NaN in eig2x2)
+    3: ValueError,        # std::domain_error
+    4: RuntimeError,      # std::logic_error (NaN in eig2x2)
     5: RuntimeError,      # c10::Error (TORCH_CHECK)
     6: MemoryError,
     7: RuntimeError,
@@ -81,6 +85,10 @@ SIGNATURES = {
     "qtt_context_sync": (C.c_int, [P]),
     "qtt_context_memory": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "qtt_context_kernel_times": (C.c_int, [P, C.c_char_p, C.c_size_t, C.c_int]),
+    "qtt_context_svd_stats": (C.c_int, [P, I64P, I64P, I64P]),
+    "qtt_context_svd_flops": (C.c_int, [P, DP, DP]),
+    "qtt_context_set_lanczos": (C.c_int, [P, C.c_int, C.c_double]),
+    "qtt_context_lanczos_stats": (C.c_int, [P, I64P, I64P]),
     "qtt_btensor_create": (C.c_int, [P, C.c_int, I64P, I64P, C.c_int, U16P, I16P, I16P, C.c_int64, I32P, DP, PP]),
     "qtt_btensor_destroy": (C.c_int, [P]),
     "qtt_btensor_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), I64P, I64P, I64P]),
@@ -100,6 +108,11 @@ SIGNATURES = {
     "qtt_truncate": (C.c_int, [P, P, P, C.c_size_t, C.c_size_t, C.c_double, C.c_double, PP, PP, PP]),
     "qtt_dmrg": (C.c_int, [C.c_int, PP, PP, C.POINTER(DmrgOptionsC), DP, C.c_int, DP, DP, I64P, C.POINTER(C.c_int),
                            DP, DP]),
+    "qtt_hamil2site_times_state": (C.c_int, [P, P, P, P, PP]),
+    "qtt_one_step_lanczos": (C.c_int, [P, P, P, P, PP, DP, DP, DP]),
+    "qtt_eig2x2": (C.c_int, [P, C.c_double, C.c_double, C.c_double, DP, DP, DP]),
+    "qtt_dmrg_from_center": (C.c_int, [C.c_int, PP, PP, C.c_int, C.POINTER(DmrgOptionsC), DP, C.c_int, DP, DP, I64P,
+                                       C.POINTER(C.c_int), DP, DP]),
     "qtt_mps_move_oc": (C.c_int, [C.c_int, PP, C.c_int, C.c_int]),
     "qtt_mps_prepare": (C.c_int, [C.c_int, PP]),
     "qtt_contract_norm": (C.c_int, [C.c_int, PP, PP, DP]),
@@ -168,6 +181,30 @@ class Context:
         u, p, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
         _check(library().qtt_context_memory(self.h, C.byref(u), C.byref(p), C.byref(c)))
         return {"used": u.value, "peak": p.value, "capacity": c.value}
+
+    def svd_flops(self):
+        """Block-SVD model FLOPs since creation: (all sectors, wide sectors)."""
+        a, w = C.c_double(), C.c_double()
+        _check(library().qtt_context_svd_flops(self.h, C.byref(a), C.byref(w)))
+        return a.value, w.value
+
+    def set_lanczos(self, max_krylov, tol=1e-12):
+        """DMRG eigensolver: 0 = the reference's one-step Lanczos (default);
+        2..64 = k-step Lanczos with full reorthogonalisation, stopping at
+        residual <= tol |E|."""
+        _check(library().qtt_context_set_lanczos(self.h, int(max_krylov), float(tol)))
+
+    def lanczos_stats(self):
+        u, v = C.c_int64(), C.c_int64()
+        _check(library().qtt_context_lanczos_stats(self.h, C.byref(u), C.byref(v)))
+        return {"updates": u.value, "vectors": v.value}
+
+    def svd_stats(self):
+        """Block SVD/eigh health: calls, largest sweep count of one call, and
+        sectors that reached the Jacobi sweep cap still rotating."""
+        c, m, u = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(library().qtt_context_svd_stats(self.h, C.byref(c), C.byref(m), C.byref(u)))
+        return {"calls": c.value, "max_sweeps": m.value, "unconverged": u.value}
 
     def kernel_times(self, reset=True):
         buf = C.create_string_buffer(1 << 16)
@@ -367,10 +404,11 @@ class DmrgOptions:
         self.maximum_iterations = maximum_iterations
 
 
-def dmrg(mpo, mps, options):
-    """``dmrg(bMPO&, bMPS&, options, logger)`` (include/dmrg.h:39-40).  The
-    state must have its orthogonality centre at site 0.  Returns
-    (energy, new mps list, log dict)."""
+def dmrg(mpo, mps, options, oc=0):
+    """``dmrg(bMPO&, bMPS&, options, logger)`` (include/dmrg.h:39-40) on a
+    state whose orthogonality centre is at site ``oc`` (moved back there at
+    the end, as the reference does).  Returns (energy, new mps list, log
+    dict)."""
     L = len(mpo)
     mpo_h = (C.c_void_p * L)(*[t.h for t in mpo])
     mps_h = (C.c_void_p * L)(*[t.h for t in mps])
@@ -387,15 +425,40 @@ def dmrg(mpo, mps, options):
     ns = C.c_int()
     cf, sf = C.c_double(), C.c_double()
     ctx = mpo[0].ctx
-    st = library().qtt_dmrg(L, C.cast(mpo_h, PP), C.cast(mps_h, PP), C.byref(o), C.byref(E), nmax,
-                            _arr(es, C.c_double), _arr(ms, C.c_double), _arr(mb, C.c_int64), C.byref(ns),
-                            C.byref(cf), C.byref(sf))
+    st = library().qtt_dmrg_from_center(L, C.cast(mpo_h, PP), C.cast(mps_h, PP), int(oc), C.byref(o), C.byref(E),
+                                        nmax, _arr(es, C.c_double), _arr(ms, C.c_double), _arr(mb, C.c_int64),
+                                        C.byref(ns), C.byref(cf), C.byref(sf))
     new_mps = [BTensor(ctx, C.c_void_p(mps_h[i])) for i in range(L)]
     _check(st)
     n = ns.value
     log = {"energies": es[:n].tolist(), "sweep_ms": ms[:n].tolist(), "max_bond": mb[:n].tolist(),
            "contract_flops": cf.value, "svd_flops": sf.value}
     return E.value, new_mps, log
+
+
+def hamil2site_times_state(theta, W12, L, R):
+    """``details::hamil2site_times_state`` (sources/dmrg.cpp:520-531)."""
+    h = C.c_void_p()
+    _check(library().qtt_hamil2site_times_state(theta.h, W12.h, L.h, R.h, C.byref(h)))
+    return BTensor(theta.ctx, h)
+
+
+def one_step_lanczos(theta, W12, L, R):
+    """``details::one_step_lanczos`` (sources/dmrg.cpp:584-608): returns
+    (Hpsi, a0, a1, b) with Hpsi the normalised Lanczos vector."""
+    h = C.c_void_p()
+    a0, a1, b = C.c_double(), C.c_double(), C.c_double()
+    _check(library().qtt_one_step_lanczos(theta.h, W12.h, L.h, R.h, C.byref(h), C.byref(a0), C.byref(a1),
+                                          C.byref(b)))
+    return BTensor(theta.ctx, h), a0.value, a1.value, b.value
+
+
+def eig2x2(ctx, a0, a1, b):
+    """``details::eig2x2Mat`` (sources/dmrg.cpp:543-572) on the device:
+    (E, o, n); a NaN raises RuntimeError (the reference's logic_error)."""
+    E, o, n = C.c_double(), C.c_double(), C.c_double()
+    _check(library().qtt_eig2x2(ctx.h, float(a0), float(a1), float(b), C.byref(E), C.byref(o), C.byref(n)))
+    return E.value, o.value, n.value
 
 
 def _replace_mps(fn, mps, *args):

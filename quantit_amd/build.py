@@ -59,9 +59,28 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_abi_client()
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     return LIB
+
+
+ABI_CLIENT_SRC = PKG.parent / "tests" / "abi_client.c"
+ABI_CLIENT = PKG.parent / "tests" / "abi_client"
+
+
+def build_abi_client() -> Path:
+    """The plain C client of include/quantit_amd.h (tests/abi_client.c),
+    linked against the library with an rpath to it."""
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        raise RuntimeError("no C compiler for the ABI client")
+    cmd = [cc, "-std=c99", "-O2", "-Wall", "-I", str(PKG.parent / "include"), str(ABI_CLIENT_SRC), "-o",
+           str(ABI_CLIENT), "-L", str(PKG), "-lquantit_amd", "-Wl,-rpath,$ORIGIN/../quantit_amd", "-lm"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"ABI client build failed\n{res.stderr}")
+    return ABI_CLIENT
 
 
 if __name__ == "__main__":

@@ -126,6 +126,11 @@ const Block* BTensor::find(const BIdx& idx) const {
 }
 
 void BTensor::allocate(const std::vector<BIdx>& idx) {
+  const int r = rank();
+  // indices are checked before any section size is read through them
+  for (auto& i : idx)
+    for (int d = 0; d < r; ++d)
+      if (i.i[d] < 0 || i.i[d] >= shape.nsec(d)) throw InvalidArgument("block index " + fmt_idx(i, r) + " out of range");
   blocks.clear();
   blocks.reserve(idx.size());
   int64_t off = 0;

@@ -101,6 +101,7 @@ double item(const BTensor& scalar);                    // synchronizing read of 
 struct SvdStats {
   double flops = 0;   // sum over sectors 14*m*n^2 + 8*n^3 (m >= n)
   int sectors = 0, sweeps = 0, max_k = 0;
+  int unconverged = 0;  // sectors whose Jacobi sweeps hit the sweep cap
 };
 struct SvdResult {
   BTensor U, d, V;

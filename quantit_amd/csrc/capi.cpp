@@ -4,17 +4,28 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "btensor.hpp"
 #include "dmrg.hpp"
+#include "kernels.hpp"
 
+// A context is deleted when it has been destroyed AND its last tensor handle
+// is gone: tensors reference their context (stream, staging), so a handle
+// that outlives qtt_context_destroy keeps the context alive, and every call
+// on such a handle except qtt_btensor_destroy fails with
+// QTT_INVALID_ARGUMENT.
 struct qtt_context {
   qtt::Context ctx;
-  qtt_context(int dev, size_t bytes) : ctx(dev, bytes) {}
+  std::mutex mu;
+  long handles = 0;
+  bool closed = false;
+  qtt_context(int dev, size_t bytes) : ctx(dev, bytes) { ctx.owner = this; }
 };
 struct qtt_btensor {
   qtt::BTensor t;
+  qtt_context* owner;
 };
 
 namespace {
@@ -43,7 +54,41 @@ void need(const void* p, const char* what) {
   if (!p) throw qtt::InvalidArgument(std::string("null pointer: ") + what);
 }
 
-qtt_btensor* wrap(qtt::BTensor&& t) { return new qtt_btensor{std::move(t)}; }
+qtt_context* owner_of(const qtt::BTensor& t) { return static_cast<qtt_context*>(t.ctx->owner); }
+
+qtt_btensor* wrap(qtt::BTensor&& t) {
+  qtt_context* c = owner_of(t);
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    ++c->handles;
+  }
+  return new qtt_btensor{std::move(t), c};
+}
+
+// The tensor behind a handle, checked: non-null and its context not destroyed.
+const qtt::BTensor& live(const qtt_btensor* h, const char* what) {
+  need(h, what);
+  std::lock_guard<std::mutex> lk(h->owner->mu);
+  if (h->owner->closed) throw qtt::InvalidArgument(std::string(what) + ": its context was destroyed");
+  return h->t;
+}
+
+// Every tensor of one call must belong to one context (one stream, one arena).
+void same_context(const qtt::BTensor& a, const qtt::BTensor& b) {
+  if (a.ctx != b.ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
+}
+
+void release(qtt_btensor* h) {
+  if (!h) return;
+  qtt_context* c = h->owner;
+  delete h;  // the tensor's storage goes back to the arena first
+  bool last = false;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    last = --c->handles == 0 && c->closed;
+  }
+  if (last) delete c;
+}
 
 }  // namespace
 
@@ -60,7 +105,17 @@ int qtt_context_create(int device, size_t arena_bytes, qtt_context** out) {
 }
 
 int qtt_context_destroy(qtt_context* ctx) {
-  return guard([&] { delete ctx; });
+  return guard([&] {
+    if (!ctx) return;
+    bool now = false;
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      if (ctx->closed) throw qtt::InvalidArgument("context destroyed twice");
+      ctx->closed = true;
+      now = ctx->handles == 0;
+    }
+    if (now) delete ctx;
+  });
 }
 
 int qtt_context_set_precision(qtt_context* c, int precision) {
@@ -128,6 +183,42 @@ int qtt_context_kernel_times(qtt_context* c, char* buf, size_t buflen, int reset
   });
 }
 
+int qtt_context_svd_stats(qtt_context* c, int64_t* calls, int64_t* max_sweeps, int64_t* unconverged) {
+  return guard([&] {
+    need(c, "ctx");
+    if (calls) *calls = c->ctx.svd_calls;
+    if (max_sweeps) *max_sweeps = c->ctx.svd_max_sweeps;
+    if (unconverged) *unconverged = c->ctx.svd_unconverged;
+  });
+}
+
+int qtt_context_svd_flops(qtt_context* c, double* all, double* wide) {
+  return guard([&] {
+    need(c, "ctx");
+    if (all) *all = c->ctx.svd_model_flops;
+    if (wide) *wide = c->ctx.svd_model_flops_wide;
+  });
+}
+
+int qtt_context_set_lanczos(qtt_context* c, int max_krylov, double tol) {
+  return guard([&] {
+    need(c, "ctx");
+    if (max_krylov != 0 && (max_krylov < 2 || max_krylov > qtt::kern::kMaxKrylov))
+      throw qtt::InvalidArgument("max_krylov must be 0 or lie in [2, " + std::to_string(qtt::kern::kMaxKrylov) + "]");
+    if (!(tol >= 0.0)) throw qtt::InvalidArgument("tolerance must be non-negative");
+    c->ctx.lanczos_krylov = max_krylov;
+    c->ctx.lanczos_tol = tol;
+  });
+}
+
+int qtt_context_lanczos_stats(qtt_context* c, int64_t* updates, int64_t* vectors) {
+  return guard([&] {
+    need(c, "ctx");
+    if (updates) *updates = c->ctx.lanczos_updates;
+    if (vectors) *vectors = c->ctx.lanczos_vectors;
+  });
+}
+
 int qtt_btensor_create(qtt_context* c, int rank, const int64_t* nsec, const int64_t* sec_sizes, int ngroups,
                        const uint16_t* group_mod, const int16_t* cvals, const int16_t* sel, int64_t nblocks,
                        const int32_t* block_idx, const double* data, qtt_btensor** out) {
@@ -136,6 +227,11 @@ int qtt_btensor_create(qtt_context* c, int rank, const int64_t* nsec, const int6
     need(out, "out");
     if (rank < 0 || rank > qtt::kMaxRank) throw qtt::InvalidArgument("rank out of range");
     if (ngroups < 1 || ngroups > qtt::kMaxGroups) throw qtt::InvalidArgument("ngroups out of range");
+    if (rank) {
+      need(nsec, "nsec");
+      for (int d = 0; d < rank; ++d)
+        if (nsec[d] < 0) throw qtt::InvalidArgument("negative section count");
+    }
     qtt::Shape s;
     s.g.n = ngroups;
     for (int i = 0; i < ngroups; ++i) s.g.mod[i] = group_mod ? group_mod[i] : 0;
@@ -155,9 +251,17 @@ int qtt_btensor_create(qtt_context* c, int rank, const int64_t* nsec, const int6
     for (int gi = 0; gi < ngroups; ++gi) s.sel.v[gi] = sel ? sel[gi] : 0;
     s.validate();
     qtt::BTensor t(&c->ctx, s);
-    std::vector<qtt::BIdx> idx(size_t(std::max<int64_t>(nblocks, 0)));
+    if (nblocks < 0) throw qtt::InvalidArgument("negative block count");
+    if (nblocks && rank) need(block_idx, "block_idx");
+    std::vector<qtt::BIdx> idx(static_cast<size_t>(nblocks));
     for (int64_t b = 0; b < nblocks; ++b)
-      for (int d = 0; d < rank; ++d) idx[b].i[d] = block_idx[b * rank + d];
+      for (int d = 0; d < rank; ++d) {
+        const int32_t k = block_idx[b * rank + d];
+        if (k < 0 || k >= s.nsec(d))
+          throw qtt::InvalidArgument("block " + std::to_string(b) + ": section index " + std::to_string(k) +
+                                     " out of range along dimension " + std::to_string(d));
+        idx[b].i[d] = k;
+      }
     int64_t n = 0;
     for (auto& b : idx) n += s.block_numel(b);
     if (n) need(data, "data");
@@ -167,28 +271,27 @@ int qtt_btensor_create(qtt_context* c, int rank, const int64_t* nsec, const int6
 }
 
 int qtt_btensor_destroy(qtt_btensor* t) {
-  return guard([&] { delete t; });
+  return guard([&] { release(t); });
 }
 
-int qtt_btensor_info(const qtt_btensor* t, int* rank, int* ngroups, int64_t* total_sections, int64_t* nblocks,
+int qtt_btensor_info(const qtt_btensor* h, int* rank, int* ngroups, int64_t* total_sections, int64_t* nblocks,
                      int64_t* numel) {
   return guard([&] {
-    need(t, "tensor");
-    if (rank) *rank = t->t.rank();
-    if (ngroups) *ngroups = t->t.shape.g.n;
+    const qtt::BTensor& t = live(h, "tensor");
+    if (rank) *rank = t.rank();
+    if (ngroups) *ngroups = t.shape.g.n;
     int64_t ts = 0;
-    for (auto& s : t->t.shape.secs) ts += int64_t(s.size());
+    for (auto& s : t.shape.secs) ts += int64_t(s.size());
     if (total_sections) *total_sections = ts;
-    if (nblocks) *nblocks = int64_t(t->t.blocks.size());
-    if (numel) *numel = t->t.numel();
+    if (nblocks) *nblocks = int64_t(t.blocks.size());
+    if (numel) *numel = t.numel();
   });
 }
 
-int qtt_btensor_structure(const qtt_btensor* t, int64_t* nsec, int64_t* sec_sizes, int16_t* cvals, int16_t* sel,
+int qtt_btensor_structure(const qtt_btensor* h, int64_t* nsec, int64_t* sec_sizes, int16_t* cvals, int16_t* sel,
                           uint16_t* group_mod) {
   return guard([&] {
-    need(t, "tensor");
-    const auto& s = t->t.shape;
+    const auto& s = live(h, "tensor").shape;
     const int ng = s.g.n;
     int64_t pos = 0;
     for (int d = 0; d < s.rank(); ++d) {
@@ -206,56 +309,64 @@ int qtt_btensor_structure(const qtt_btensor* t, int64_t* nsec, int64_t* sec_size
   });
 }
 
-int qtt_btensor_blocks(const qtt_btensor* t, int32_t* block_idx, double* data) {
+int qtt_btensor_blocks(const qtt_btensor* h, int32_t* block_idx, double* data) {
   return guard([&] {
-    need(t, "tensor");
-    const int r = t->t.rank();
+    const qtt::BTensor& t = live(h, "tensor");
+    const int r = t.rank();
     if (block_idx)
-      for (size_t b = 0; b < t->t.blocks.size(); ++b)
-        for (int d = 0; d < r; ++d) block_idx[b * r + d] = t->t.blocks[b].idx.i[d];
+      for (size_t b = 0; b < t.blocks.size(); ++b)
+        for (int d = 0; d < r; ++d) block_idx[b * r + d] = t.blocks[b].idx.i[d];
     if (data) {
-      auto h = t->t.download();
-      if (!h.empty()) std::memcpy(data, h.data(), h.size() * sizeof(double));
+      auto hd = t.download();
+      if (!hd.empty()) std::memcpy(data, hd.data(), hd.size() * sizeof(double));
     }
   });
 }
 
-int qtt_permute(const qtt_btensor* t, const int64_t* perm, qtt_btensor** out) {
+int qtt_permute(const qtt_btensor* h, const int64_t* perm, qtt_btensor** out) {
   return guard([&] {
-    need(t, "tensor");
+    const qtt::BTensor& t = live(h, "tensor");
     need(out, "out");
-    std::vector<int64_t> p(perm, perm + t->t.rank());
-    *out = wrap(qtt::permute(t->t, p));
+    if (t.rank()) need(perm, "perm");
+    std::vector<int64_t> p(perm, perm + t.rank());
+    *out = wrap(qtt::permute(t, p));
   });
 }
 
-int qtt_reshape(const qtt_btensor* t, int ngroups, const int64_t* groups, qtt_btensor** out) {
+int qtt_reshape(const qtt_btensor* h, int ngroups, const int64_t* groups, qtt_btensor** out) {
   return guard([&] {
-    need(t, "tensor");
+    const qtt::BTensor& t = live(h, "tensor");
     need(out, "out");
+    if (ngroups < 0) throw qtt::InvalidArgument("negative group count");
+    if (ngroups) need(groups, "index_groups");
     std::vector<int64_t> g(groups, groups + ngroups);
-    *out = wrap(qtt::reshape(t->t, g));
+    *out = wrap(qtt::reshape(t, g));
   });
 }
 
-int qtt_conj(const qtt_btensor* t, qtt_btensor** out) {
+int qtt_conj(const qtt_btensor* h, qtt_btensor** out) {
   return guard([&] {
-    need(t, "tensor");
+    const qtt::BTensor& t = live(h, "tensor");
     need(out, "out");
-    *out = wrap(qtt::conj(t->t));
+    *out = wrap(qtt::conj(t));
   });
 }
 
 int qtt_tensordot(const qtt_btensor* a, const qtt_btensor* b, int ndims, const int64_t* dims_a, const int64_t* dims_b,
                   qtt_btensor** out) {
   return guard([&] {
-    need(a, "a");
-    need(b, "b");
+    const qtt::BTensor& ta = live(a, "a");
+    const qtt::BTensor& tb = live(b, "b");
     need(out, "out");
-    if (a->t.ctx != b->t.ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
+    same_context(ta, tb);
+    if (ndims < 0) throw qtt::InvalidArgument("negative number of contracted dimensions");
+    if (ndims) {
+      need(dims_a, "dims_a");
+      need(dims_b, "dims_b");
+    }
     std::vector<int64_t> da(dims_a, dims_a + ndims), db(dims_b, dims_b + ndims);
     qtt::ContractStats st;
-    *out = wrap(qtt::tensordot(a->t, b->t, da, db, &st));
+    *out = wrap(qtt::tensordot(ta, tb, da, db, &st));
     g_last_flops = st.flops;
   });
 }
@@ -264,38 +375,41 @@ double qtt_last_tensordot_flops(void) { return g_last_flops; }
 
 int qtt_mul(const qtt_btensor* x, const qtt_btensor* y, qtt_btensor** out) {
   return guard([&] {
-    need(x, "x");
-    need(y, "y");
+    const qtt::BTensor& tx = live(x, "x");
+    const qtt::BTensor& ty = live(y, "y");
     need(out, "out");
-    *out = wrap(qtt::mul(x->t, y->t));
+    same_context(tx, ty);
+    *out = wrap(qtt::mul(tx, ty));
   });
 }
 
 int qtt_add(const qtt_btensor* x, const qtt_btensor* y, double alpha, qtt_btensor** out) {
   return guard([&] {
-    need(x, "x");
-    need(y, "y");
+    const qtt::BTensor& tx = live(x, "x");
+    const qtt::BTensor& ty = live(y, "y");
     need(out, "out");
-    *out = wrap(qtt::add(x->t, y->t, alpha));
+    same_context(tx, ty);
+    *out = wrap(qtt::add(tx, ty, alpha));
   });
 }
 
 int qtt_full_dot(const qtt_btensor* x, const qtt_btensor* y, double* out) {
   return guard([&] {
-    need(x, "x");
-    need(y, "y");
+    const qtt::BTensor& tx = live(x, "x");
+    const qtt::BTensor& ty = live(y, "y");
     need(out, "out");
-    *out = qtt::item(qtt::full_dot(x->t, y->t));
+    same_context(tx, ty);
+    *out = qtt::item(qtt::full_dot(tx, ty));
   });
 }
 
 int qtt_svd(const qtt_btensor* t, size_t split, qtt_btensor** U, qtt_btensor** d, qtt_btensor** V) {
   return guard([&] {
-    need(t, "tensor");
+    const qtt::BTensor& tt = live(t, "tensor");
     need(U, "U");
     need(d, "d");
     need(V, "V");
-    auto r = qtt::svd(t->t, split);
+    auto r = qtt::svd(tt, split);
     *U = wrap(std::move(r.U));
     *d = wrap(std::move(r.d));
     *V = wrap(std::move(r.V));
@@ -305,11 +419,11 @@ int qtt_svd(const qtt_btensor* t, size_t split, qtt_btensor** U, qtt_btensor** d
 int qtt_svd_truncated(const qtt_btensor* t, size_t split, double tol, size_t min_size, size_t max_size, double pow,
                       qtt_btensor** U, qtt_btensor** d, qtt_btensor** V) {
   return guard([&] {
-    need(t, "tensor");
+    const qtt::BTensor& tt = live(t, "tensor");
     need(U, "U");
     need(d, "d");
     need(V, "V");
-    auto r = qtt::svd_truncated(t->t, split, tol, min_size, max_size, pow);
+    auto r = qtt::svd_truncated(tt, split, tol, min_size, max_size, pow);
     *U = wrap(std::move(r.U));
     *d = wrap(std::move(r.d));
     *V = wrap(std::move(r.V));
@@ -318,10 +432,10 @@ int qtt_svd_truncated(const qtt_btensor* t, size_t split, double tol, size_t min
 
 int qtt_eigh(const qtt_btensor* t, size_t split, qtt_btensor** d, qtt_btensor** U) {
   return guard([&] {
-    need(t, "tensor");
+    const qtt::BTensor& tt = live(t, "tensor");
     need(U, "U");
     need(d, "d");
-    auto r = qtt::eigh(t->t, split);
+    auto r = qtt::eigh(tt, split);
     *d = wrap(std::move(r.d));
     *U = wrap(std::move(r.U));
   });
@@ -330,13 +444,26 @@ int qtt_eigh(const qtt_btensor* t, size_t split, qtt_btensor** d, qtt_btensor** 
 int qtt_truncate(const qtt_btensor* U, const qtt_btensor* d, const qtt_btensor* V, size_t max_size, size_t min_size,
                  double tol, double pow, qtt_btensor** U_out, qtt_btensor** d_out, qtt_btensor** V_out) {
   return guard([&] {
-    need(U, "U");
-    need(d, "d");
-    need(V, "V");
+    const qtt::BTensor& tu = live(U, "U");
+    const qtt::BTensor& td = live(d, "d");
+    const qtt::BTensor& tv = live(V, "V");
     need(U_out, "U_out");
     need(d_out, "d_out");
     need(V_out, "V_out");
-    qtt::SvdResult in{U->t, d->t, V->t};
+    same_context(tu, td);
+    same_context(tu, tv);
+    // the shapes an svd returns: d rank 1, and the last legs of U and V carry
+    // d's sections (the truncation resizes them section by section)
+    if (td.rank() != 1) throw qtt::InvalidArgument("truncate: d must have rank 1");
+    if (tu.rank() < 1 || tv.rank() < 1) throw qtt::InvalidArgument("truncate: U and V need at least one dimension");
+    const int nd = td.shape.nsec(0);
+    if (tu.shape.nsec(tu.rank() - 1) != nd || tv.shape.nsec(tv.rank() - 1) != nd)
+      throw qtt::InvalidArgument("truncate: the last dimensions of U and V must have the sections of d");
+    for (auto& b : td.blocks)
+      if (tu.shape.secs[tu.rank() - 1][b.idx.i[0]] != td.shape.secs[0][b.idx.i[0]] ||
+          tv.shape.secs[tv.rank() - 1][b.idx.i[0]] != td.shape.secs[0][b.idx.i[0]])
+        throw qtt::InvalidArgument("truncate: section sizes of U, d and V differ");
+    qtt::SvdResult in{tu, td, tv};
     auto r = qtt::truncate(std::move(in), max_size, min_size, tol, pow);
     *U_out = wrap(std::move(r.U));
     *d_out = wrap(std::move(r.d));
@@ -344,20 +471,55 @@ int qtt_truncate(const qtt_btensor* U, const qtt_btensor* d, const qtt_btensor* 
   });
 }
 
+namespace {
+
+// Tensors of an MPS/MPO list, all of one context.
+std::vector<qtt::BTensor> gather(int L, qtt_btensor* const* hs, const char* what, const qtt::Context* ctx) {
+  need(hs, what);
+  std::vector<qtt::BTensor> v;
+  for (int i = 0; i < L; ++i) {
+    const qtt::BTensor& t = live(hs[i], what);
+    if (ctx && t.ctx != ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
+    v.push_back(t);
+  }
+  return v;
+}
+
+void check_length(int L) {
+  if (L < 1) throw qtt::InvalidArgument("an MPS needs at least one site");
+}
+
+// Replaces the handles of a list in place (the old ones are released).
+void replace(int L, qtt_btensor** hs, std::vector<qtt::BTensor>& v) {
+  for (int i = 0; i < L; ++i) {
+    qtt_btensor* n = wrap(std::move(v[i]));
+    release(hs[i]);
+    hs[i] = n;
+  }
+}
+
+}  // namespace
+
 int qtt_dmrg(int L, qtt_btensor* const* mpo, qtt_btensor** mps, const qtt_dmrg_options* options, double* energy,
              int max_log, double* sweep_energies, double* sweep_ms, int64_t* max_bond, int* n_sweeps,
              double* contract_flops, double* svd_flops) {
+  return qtt_dmrg_from_center(L, mpo, mps, 0, options, energy, max_log, sweep_energies, sweep_ms, max_bond, n_sweeps,
+                              contract_flops, svd_flops);
+}
+
+int qtt_dmrg_from_center(int L, qtt_btensor* const* mpo, qtt_btensor** mps, int oc, const qtt_dmrg_options* options,
+                         double* energy, int max_log, double* sweep_energies, double* sweep_ms, int64_t* max_bond,
+                         int* n_sweeps, double* contract_flops, double* svd_flops) {
   return guard([&] {
-    need(mpo, "mpo");
-    need(mps, "mps");
     need(options, "options");
+    check_length(L);
     qtt::DmrgState st;
-    for (int i = 0; i < L; ++i) {
-      need(mpo[i], "mpo[i]");
-      need(mps[i], "mps[i]");
-      st.mpo.push_back(mpo[i]->t);
-      st.mps.push_back(mps[i]->t);
-    }
+    st.oc = oc;
+    need(mps, "mps");
+    st.mps = gather(L, mps, "mps", nullptr);
+    st.mpo = gather(L, mpo, "mpo", st.mps[0].ctx);
+    for (auto& t : st.mps)
+      if (t.ctx != st.mps[0].ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
     qtt::DmrgOptions opt;
     opt.cutoff = options->cutoff;
     opt.convergence_criterion = options->convergence_criterion;
@@ -366,10 +528,7 @@ int qtt_dmrg(int L, qtt_btensor* const* mpo, qtt_btensor** mps, const qtt_dmrg_o
     opt.maximum_iterations = size_t(options->maximum_iterations);
     qtt::DmrgLog log;
     const double E = qtt::dmrg_run(st, opt, &log);
-    for (int i = 0; i < L; ++i) {
-      delete mps[i];
-      mps[i] = wrap(std::move(st.mps[i]));
-    }
+    replace(L, mps, st.mps);
     if (energy) *energy = E;
     const int ns = int(log.energies.size());
     if (n_sweeps) *n_sweeps = ns;
@@ -383,65 +542,101 @@ int qtt_dmrg(int L, qtt_btensor* const* mpo, qtt_btensor** mps, const qtt_dmrg_o
   });
 }
 
+int qtt_hamil2site_times_state(const qtt_btensor* theta, const qtt_btensor* W12, const qtt_btensor* L,
+                               const qtt_btensor* R, qtt_btensor** out) {
+  return guard([&] {
+    const qtt::BTensor& t = live(theta, "theta");
+    const qtt::BTensor& w = live(W12, "W12");
+    const qtt::BTensor& l = live(L, "L");
+    const qtt::BTensor& r = live(R, "R");
+    need(out, "out");
+    same_context(t, w);
+    same_context(t, l);
+    same_context(t, r);
+    *out = wrap(qtt::hamil2site_times_state(t, w, l, r));
+  });
+}
+
+int qtt_one_step_lanczos(const qtt_btensor* theta, const qtt_btensor* W12, const qtt_btensor* L,
+                         const qtt_btensor* R, qtt_btensor** Hpsi, double* a0, double* a1, double* b) {
+  return guard([&] {
+    const qtt::BTensor& t = live(theta, "theta");
+    const qtt::BTensor& w = live(W12, "W12");
+    const qtt::BTensor& l = live(L, "L");
+    const qtt::BTensor& r = live(R, "R");
+    need(Hpsi, "Hpsi");
+    same_context(t, w);
+    same_context(t, l);
+    same_context(t, r);
+    qtt::LanczosStep st = qtt::one_step_lanczos(t, w, l, r);
+    if (a0) *a0 = qtt::item(st.a0);
+    if (a1) *a1 = qtt::item(st.a1);
+    if (b) *b = qtt::item(st.b);
+    *Hpsi = wrap(std::move(st.Hpsi));
+  });
+}
+
+int qtt_eig2x2(qtt_context* c, double a0, double a1, double b, double* E, double* o, double* n) {
+  return guard([&] {
+    need(c, "ctx");
+    qtt::Shape s0;
+    s0.sel = qtt::q_neutral();
+    qtt::BTensor x0 = qtt::fill_like(&c->ctx, s0, a0), x1 = qtt::fill_like(&c->ctx, s0, a1),
+                 xb = qtt::fill_like(&c->ctx, s0, b);
+    qtt::Eig2x2 r = qtt::eig2x2_dev(x0, x1, xb);
+    int32_t flag = 0;
+    c->ctx.read_small(&flag, r.flag->cptr(), sizeof(int32_t));
+    if (flag) throw qtt::LogicError("nan found in output tensor");
+    if (E) *E = qtt::item(r.E);
+    if (o) *o = qtt::item(r.o);
+    if (n) *n = qtt::item(r.n);
+  });
+}
+
 int qtt_contract_norm(int L, qtt_btensor* const* a, qtt_btensor* const* b, double* out) {
   return guard([&] {
-    need(a, "a");
-    need(b, "b");
     need(out, "out");
-    std::vector<qtt::BTensor> va, vb;
-    for (int i = 0; i < L; ++i) {
-      va.push_back(a[i]->t);
-      vb.push_back(b[i]->t);
-    }
+    check_length(L);
+    auto va = gather(L, a, "a", nullptr);
+    auto vb = gather(L, b, "b", va[0].ctx);
+    for (auto& t : va)
+      if (t.ctx != va[0].ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
     *out = qtt::contract_norm(va, vb);
   });
 }
 
 int qtt_contract_energy(int L, qtt_btensor* const* a, qtt_btensor* const* b, qtt_btensor* const* mpo, double* out) {
   return guard([&] {
-    need(a, "a");
-    need(b, "b");
-    need(mpo, "mpo");
     need(out, "out");
-    std::vector<qtt::BTensor> va, vb, vm;
-    for (int i = 0; i < L; ++i) {
-      va.push_back(a[i]->t);
-      vb.push_back(b[i]->t);
-      vm.push_back(mpo[i]->t);
-    }
+    check_length(L);
+    auto va = gather(L, a, "a", nullptr);
+    auto vb = gather(L, b, "b", va[0].ctx);
+    auto vm = gather(L, mpo, "mpo", va[0].ctx);
+    for (auto& t : va)
+      if (t.ctx != va[0].ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
     *out = qtt::contract_energy(va, vb, vm);
   });
 }
 
 int qtt_mps_move_oc(int L, qtt_btensor** mps, int oc, int target) {
   return guard([&] {
-    need(mps, "mps");
-    std::vector<qtt::BTensor> v;
-    for (int i = 0; i < L; ++i) {
-      need(mps[i], "mps[i]");
-      v.push_back(mps[i]->t);
-    }
+    check_length(L);
+    auto v = gather(L, mps, "mps", nullptr);
+    for (auto& t : v)
+      if (t.ctx != v[0].ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
     qtt::mps_move_oc(v, oc, target);
-    for (int i = 0; i < L; ++i) {
-      delete mps[i];
-      mps[i] = wrap(std::move(v[i]));
-    }
+    replace(L, mps, v);
   });
 }
 
 int qtt_mps_prepare(int L, qtt_btensor** mps) {
   return guard([&] {
-    need(mps, "mps");
-    std::vector<qtt::BTensor> v;
-    for (int i = 0; i < L; ++i) {
-      need(mps[i], "mps[i]");
-      v.push_back(mps[i]->t);
-    }
+    check_length(L);
+    auto v = gather(L, mps, "mps", nullptr);
+    for (auto& t : v)
+      if (t.ctx != v[0].ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
     qtt::mps_prepare(v);
-    for (int i = 0; i < L; ++i) {
-      delete mps[i];
-      mps[i] = wrap(std::move(v[i]));
-    }
+    replace(L, mps, v);
   });
 }
 

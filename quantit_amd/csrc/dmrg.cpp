@@ -98,9 +98,8 @@ BTensor eye_like(Context* ctx, const Shape& s) {
 void dmrg_prepare(DmrgState& st) {
   const int L = int(st.mpo.size());
   if (L < 2 || int(st.mps.size()) != L) throw InvalidArgument("MPS and MPO must have the same length >= 2");
+  if (st.oc < 0 || st.oc >= L) throw InvalidArgument("orthogonality center outside the MPS");
   Context* ctx = st.mps[0].ctx;
-  st.oc = 0;
-  st.step = 1;
   st.env.assign(size_t(L + 2), BTensor());
   Shape Ls = edge_shape_prep(st.mps.front().shape, 0);
   Shape LH = edge_shape_prep(st.mpo.front().shape, 0);
@@ -110,74 +109,73 @@ void dmrg_prepare(DmrgState& st) {
   Shape RH = edge_shape_prep(st.mpo.back().shape, 2);
   Shape Rsi = Rs.inverse_cvals();
   env_at(st, L) = fill_like(ctx, shape_product({&Rs, &RH, &Rsi}), 1.0);
+  // generate_env_impl (dmrg.cpp:370-409): left environments below the
+  // orthogonality centre, right environments above it
+  for (int i = 0; i < st.oc; ++i) env_at(st, i) = compute_left_env(st.mpo[i], st.mps[i], env_at(st, i - 1));
   for (int i = L - 1; i > st.oc; --i) env_at(st, i) = compute_right_env(st.mpo[i], st.mps[i], env_at(st, i + 1));
   st.w12.clear();
   for (int i = 0; i + 1 < L; ++i)
     st.w12.push_back(permute(tensordot(st.mpo[i], st.mpo[i + 1], {2}, {0}), {0, 1, 3, 4, 2, 5}));
 }
 
-BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
-  const int oc = st.oc;
-  Context* ctx = st.mps[0].ctx;
-  const Groups& g = st.mps[0].shape.g;
-  ContractStats cs;
-  SvdStats ss;
-  BTensor& Lenv = env_at(st, oc - 1);
-  BTensor& Renv = env_at(st, oc + 2);
-  const BTensor& W = st.w12[oc];
-  BTensor theta = tensordot(st.mps[oc], st.mps[oc + 1], {2}, {0}, &cs);
-  // one_step_lanczos (dmrg.cpp:584-608)
-  auto matvec = [&](const BTensor& x) {
-    BTensor out;
-    {
-      GemmLabel gl(ctx, "gemm_mv_L");
-      out = tensordot(Lenv, x, {0}, {0}, &cs);
-    }
-    {
-      GemmLabel gl(ctx, "gemm_mv_W");
-      out = tensordot(out, W, {0, 2, 3}, {0, 4, 5}, &cs);
-    }
-    GemmLabel gl(ctx, "gemm_mv_R");
-    return tensordot(out, Renv, {1, 4}, {0, 1}, &cs);
-  };
-  BTensor psi = matvec(theta);
-  BTensor a0 = full_dot(psi, theta);
-  if (a0.blocks.empty()) a0 = zero_scalar(ctx, g);
-  else psi = add(psi, mul(theta, a0), -1.0);
-  BTensor b = full_dot(psi, psi, true);
-  if (b.blocks.empty()) b = zero_scalar(ctx, g);
-  psi = div_scalar_tensor(psi, b, 1e-15);
-  BTensor a1 = full_dot(matvec(psi), psi);
-  if (a1.blocks.empty()) a1 = zero_scalar(ctx, g);
-  // eig2x2Mat (dmrg.cpp:543-572) on device
+BTensor matvec_labelled(const BTensor& x, const BTensor& W, const BTensor& Lenv, const BTensor& Renv,
+                        ContractStats* cs) {
+  Context* ctx = x.ctx;
+  BTensor out;
+  {
+    GemmLabel gl(ctx, "gemm_mv_L");
+    out = tensordot(Lenv, x, {0}, {0}, cs);
+  }
+  {
+    GemmLabel gl(ctx, "gemm_mv_W");
+    out = tensordot(out, W, {0, 2, 3}, {0, 4, 5}, cs);
+  }
+  GemmLabel gl(ctx, "gemm_mv_R");
+  return tensordot(out, Renv, {1, 4}, {0, 1}, cs);
+}
+
+LanczosStep one_step_lanczos(const BTensor& theta, const BTensor& W, const BTensor& Lenv, const BTensor& Renv,
+                             ContractStats* cs) {
+  Context* ctx = theta.ctx;
+  const Groups& g = theta.shape.g;
+  LanczosStep r;
+  BTensor psi = matvec_labelled(theta, W, Lenv, Renv, cs);
+  r.a0 = full_dot(psi, theta);
+  if (r.a0.blocks.empty()) r.a0 = zero_scalar(ctx, g);
+  else psi = add(psi, mul(theta, r.a0), -1.0);
+  r.b = full_dot(psi, psi, true);
+  if (r.b.blocks.empty()) r.b = zero_scalar(ctx, g);
+  r.Hpsi = div_scalar_tensor(psi, r.b, 1e-15);
+  r.a1 = full_dot(matvec_labelled(r.Hpsi, W, Lenv, Renv, cs), r.Hpsi);
+  if (r.a1.blocks.empty()) r.a1 = zero_scalar(ctx, g);
+  return r;
+}
+
+Eig2x2 eig2x2_dev(const BTensor& a0, const BTensor& a1, const BTensor& b) {
+  Context* ctx = a0.ctx;
   Shape s0;
-  s0.g = g;
+  s0.g = a0.shape.g;
   s0.sel = q_neutral();
   BufPtr trio = ctx->alloc(4 * sizeof(double));
   double* ab = reinterpret_cast<double*>(ctx->arena().base());
-  BTensor E(ctx, s0), o(ctx, s0), n(ctx, s0);
-  for (BTensor* x : {&E, &o, &n}) {
+  Eig2x2 r{BTensor(ctx, s0), BTensor(ctx, s0), BTensor(ctx, s0), nullptr};
+  for (BTensor* x : {&r.E, &r.o, &r.n}) {
     x->storage = trio;
     x->blocks.push_back({BIdx{}, 0, 1});
   }
-  o.blocks[0].off = 1;
-  n.blocks[0].off = 2;
-  BufPtr flag = ctx->alloc(sizeof(int));
-  kern::fill_i32(ctx->stream(), reinterpret_cast<int32_t*>(flag->cptr()), 1, 0);
-  {
-    KernelTimer kt(*ctx, "eig2x2");
-    kern::eig2x2(ctx->stream(), ab, a0.ptr(a0.blocks[0]) - ab, a1.ptr(a1.blocks[0]) - ab, b.ptr(b.blocks[0]) - ab,
-                 trio->dptr() - ab, reinterpret_cast<int*>(flag->cptr()));
-  }
-  // the flag lands in pinned memory without a wait; the truncation's read of
-  // its kept counts synchronizes the stream after it
-  int32_t* h_flag = ctx->pinned_flag();
-  *h_flag = 0;
-  QTT_HIP(hipMemcpyAsync(h_flag, flag->cptr(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
-  BTensor theta2 = axpby_dev(theta, o, psi, n);
-  SvdResult r = svd_truncated(theta2, 2, opt.cutoff, opt.minimum_bond, opt.maximum_bond, 2.0, &ss);
-  ctx->sync();  // normally already done by the truncation; the flag copy is complete
-  if (*h_flag) throw LogicError("nan found in output tensor");
+  r.o.blocks[0].off = 1;
+  r.n.blocks[0].off = 2;
+  r.flag = ctx->alloc(sizeof(int));
+  kern::fill_i32(ctx->stream(), reinterpret_cast<int32_t*>(r.flag->cptr()), 1, 0);
+  KernelTimer kt(*ctx, "eig2x2");
+  kern::eig2x2(ctx->stream(), ab, a0.ptr(a0.blocks[0]) - ab, a1.ptr(a1.blocks[0]) - ab, b.ptr(b.blocks[0]) - ab,
+               trio->dptr() - ab, reinterpret_cast<int*>(r.flag->cptr()));
+  return r;
+}
+
+// The post-SVD part of the update (dmrg.cpp:188-201): d normalised, absorbed
+// into V (forward) or U (backward), environment of the site left behind.
+static void absorb(DmrgState& st, SvdResult& r, int oc) {
   normalize_d(r.d);
   if (st.step == 1) {
     st.mps[oc] = r.U;
@@ -188,6 +186,48 @@ BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
     st.mps[oc + 1] = permute(conj(r.V), {2, 0, 1});
     env_at(st, oc + 1) = compute_right_env(st.mpo[oc + 1], st.mps[oc + 1], env_at(st, oc + 2));
   }
+}
+
+BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
+  const int oc = st.oc;
+  Context* ctx = st.mps[0].ctx;
+  ContractStats cs;
+  SvdStats ss;
+  BTensor& Lenv = env_at(st, oc - 1);
+  BTensor& Renv = env_at(st, oc + 2);
+  const BTensor& W = st.w12[oc];
+  BTensor theta = tensordot(st.mps[oc], st.mps[oc + 1], {2}, {0}, &cs);
+  if (ctx->lanczos_krylov >= 2) {
+    LanczosResult lr = lanczos_ground(theta, W, Lenv, Renv, ctx->lanczos_krylov, ctx->lanczos_tol, &cs);
+    SvdResult r = svd_truncated(lr.theta, 2, opt.cutoff, opt.minimum_bond, opt.maximum_bond, 2.0, &ss);
+    absorb(st, r, oc);
+    if (log) {
+      log->contract_flops += cs.flops;
+      log->svd_flops += ss.flops;
+      log->updates += 1;
+      log->lanczos_vectors += lr.iterations;
+    }
+    ctx->lanczos_updates += 1;
+    ctx->lanczos_vectors += lr.iterations;
+    return lr.E;
+  }
+  // one_step_lanczos (dmrg.cpp:584-608), eig2x2Mat (dmrg.cpp:543-572) on device
+  LanczosStep ls = one_step_lanczos(theta, W, Lenv, Renv, &cs);
+  Eig2x2 eg = eig2x2_dev(ls.a0, ls.a1, ls.b);
+  const BTensor& psi = ls.Hpsi;
+  const BTensor& E = eg.E;
+  const BTensor& o = eg.o;
+  const BTensor& n = eg.n;
+  // the flag lands in pinned memory without a wait; the truncation's read of
+  // its kept counts synchronizes the stream after it
+  int32_t* h_flag = ctx->pinned_flag();
+  *h_flag = 0;
+  QTT_HIP(hipMemcpyAsync(h_flag, eg.flag->cptr(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream()));
+  BTensor theta2 = axpby_dev(theta, o, psi, n);
+  SvdResult r = svd_truncated(theta2, 2, opt.cutoff, opt.minimum_bond, opt.maximum_bond, 2.0, &ss);
+  ctx->sync();  // normally already done by the truncation; the flag copy is complete
+  if (*h_flag) throw LogicError("nan found in output tensor");
+  absorb(st, r, oc);
   if (log) {
     log->contract_flops += cs.flops;
     log->svd_flops += ss.flops;
@@ -210,6 +250,16 @@ double dmrg_sweep(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
 
 double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
   dmrg_prepare(st);
+  const int L = int(st.mpo.size());
+  // details::dmrg_impl (dmrg.cpp:224-236): sweeps start toward the far end
+  // and come back; a centre on the last site starts one site to the left
+  int init_pos = st.oc;
+  st.step = st.oc == 0 ? 1 : -1;
+  if (st.w12.size() == 1) st.step = 0;
+  if (st.oc == L - 1) {
+    --init_pos;
+    --st.oc;
+  }
   double E0 = 100000.0;
   for (size_t it = 0; it < opt.maximum_iterations; ++it) {
     auto t0 = std::chrono::steady_clock::now();
@@ -226,6 +276,13 @@ double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
     const double Eold = E0;
     E0 = E;
     if (!(std::fabs((E0 - Eold) / E0) > opt.convergence_criterion)) break;
+  }
+  // dmrg.cpp:257-266: back to where the centre started
+  if (st.oc != init_pos) {
+    if (st.oc != init_pos - 1 && init_pos != L - 1)
+      throw Error(kRuntimeError, "the orthogonality center finished somewhere surprising! final oc: " +
+                                     std::to_string(st.oc) + ". original oc: " + std::to_string(init_pos));
+    mps_move_oc(st.mps, st.oc, init_pos);
   }
   return E0;
 }

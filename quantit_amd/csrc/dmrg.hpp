@@ -25,11 +25,12 @@ struct DmrgLog {
   std::vector<int64_t> middle_bond;
   double contract_flops = 0, svd_flops = 0;
   int64_t updates = 0;
+  int64_t lanczos_vectors = 0;  // Krylov vectors built (k-step Lanczos mode)
 };
 
 struct DmrgState {
   std::vector<BTensor> mpo;  // W_i (l, s, r, s')
-  std::vector<BTensor> mps;  // A_i (l, s, r), orthogonality centre at 0
+  std::vector<BTensor> mps;  // A_i (l, s, r), orthogonality centre at oc
   std::vector<BTensor> w12;  // two-site Hamiltonians
   std::vector<BTensor> env;  // env[i + 1] = Env[i] for i = -1..L
   int oc = 0;
@@ -44,7 +45,8 @@ void dmrg_prepare(DmrgState& st);
 BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
 // One full sweep: 2(L-2) updates (dmrg.cpp:127-142); returns the last energy.
 double dmrg_sweep(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
-// details::dmrg_impl (dmrg.cpp:219-273) from orthogonality centre 0.
+// details::dmrg_impl (dmrg.cpp:219-273) from the orthogonality centre
+// st.oc; the centre is moved back there at the end.
 double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
 
 // Environment helpers (dmrg.cpp:334-345, 424-493).
@@ -52,6 +54,39 @@ Shape edge_shape_prep(const Shape& t, int dim);
 BTensor compute_left_env(const BTensor& W, const BTensor& Y, const BTensor& L);
 BTensor compute_right_env(const BTensor& W, const BTensor& Y, const BTensor& R);
 BTensor hamil2site_times_state(const BTensor& theta, const BTensor& W12, const BTensor& L, const BTensor& R);
+
+// one_step_lanczos (sources/dmrg.cpp:584-608): Hpsi is the normalised
+// Lanczos vector psi1 = (H theta - a0 theta) / b (not divided when |b| <
+// 1e-15), a0 = <theta|H theta>, b = |H theta - a0 theta|, a1 = <psi1|H psi1>,
+// all rank-0 tensors on device.
+struct LanczosStep {
+  BTensor Hpsi, a0, a1, b;
+};
+LanczosStep one_step_lanczos(const BTensor& theta, const BTensor& W12, const BTensor& L, const BTensor& R,
+                             ContractStats* cs = nullptr);
+// eig2x2Mat (sources/dmrg.cpp:543-572) on device: the lowest eigenvalue E of
+// [[a0, b], [b, a1]] and its eigenvector (o, n).  flag (one int) becomes 1
+// when a NaN was met; the caller reads it (the reference throws
+// std::logic_error).
+struct Eig2x2 {
+  BTensor E, o, n;
+  BufPtr flag;
+};
+Eig2x2 eig2x2_dev(const BTensor& a0, const BTensor& a1, const BTensor& b);
+
+// k-step Lanczos with full reorthogonalisation (FP64, on device) for the
+// lowest eigenpair of the two-site Hamiltonian L.W.R: stops when the residual
+// |H theta' - E theta'| is at most tol |E|, at an invariant subspace, or after
+// max_krylov vectors.  With max_krylov = 2 it is the reference's one-step
+// update (the 2x2 problem of one_step_lanczos and eig2x2Mat).
+struct LanczosResult {
+  BTensor theta;  // normalised Ritz vector
+  BTensor E;      // rank-0 Ritz value on device
+  double energy = 0, residual = 0;
+  int iterations = 0;
+};
+LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W12, const BTensor& L, const BTensor& R,
+                             int max_krylov, double tol, ContractStats* cs = nullptr);
 
 // bMPS::move_oc (sources/MPT.cpp:74-111): moves the orthogonality centre from
 // oc to target with untruncated block SVDs; oc is updated.

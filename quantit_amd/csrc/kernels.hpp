@@ -35,6 +35,7 @@ constexpr int kGemmTile = 64;
 
 void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
               const GemmPair* pairs, double alpha, double beta);
+
 // hi/lo split FP32 MFMA: each operand x = hi + lo (hi = fp32(x), lo =
 // fp32(x - hi)); products hi*hi + hi*lo + lo*hi accumulate in FP32 per K
 // chunk and are folded into FP64 accumulators after every chunk.
@@ -119,6 +120,24 @@ void eig2x2(hipStream_t s, double* base, int64_t a0_off, int64_t a1_off, int64_t
             int* flags);
 // d /= sqrt(sum d^2) over n contiguous values.
 void normalize(hipStream_t s, double* base, int64_t off, int64_t n);
+
+// ---- k-step Lanczos (kernels/lanczos.hip) -----------------------------------
+// Krylov vectors are the rows of V (row pitch ld doubles, n used).
+constexpr int kMaxKrylov = 64;
+constexpr int kKrylovPerThread = 8;
+constexpr int kKrylovChunk = 256 * kKrylovPerThread;
+// out[i] = <V[i], w> for i < m (partials: m * ceil(n / kKrylovChunk) doubles).
+void krylov_dot(hipStream_t s, double* base, int64_t v_off, int64_t ld, int m, int64_t w_off, int64_t n,
+                int64_t partial_off, int64_t out_off);
+// dst = src + sign * sum_i coef[i] V[i] (src_off < 0: no src term), divided by
+// sqrt(base[scale_off]) when scale_off >= 0.
+void krylov_axpy(hipStream_t s, double* base, int64_t v_off, int64_t ld, int m, int64_t coef_off, double sign,
+                 int64_t src_off, int64_t dst_off, int64_t n, int64_t scale_off);
+// alpha[j] = c1[j] + c2[j], beta[j] = sqrt(nrm2) into T (alpha T[0..K), beta
+// T[K..2K)), then the lowest eigenpair of the leading (j+1)-square
+// tridiagonal: out = {eigenvalue, residual beta[j] |y[j]|, beta[j], y...}.
+void lanczos_tridiag(hipStream_t s, double* base, int64_t t_off, int K, int j, int64_t c1_off, int64_t c2_off,
+                     int64_t nrm2_off, int64_t out_off, int* flags);
 
 // ---- block SVD ----------------------------------------------------------
 // Sector work matrices: Y (ell x k, column-major, ld = ell) is orthogonalised

@@ -429,18 +429,20 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
 
 // Rows of a block pair that stay resident in LDS (dynamic size at most the
 // device's per-block LDS less the static arrays).
+// (Thread-safe static initialisation: concurrent solves on several host
+// threads see the limit only after the attribute has been set.)
 int svd_round_dyn_max() {
-  static int dyn_max = -1;
-  if (dyn_max < 0) {
+  static const int dyn_max = [] {
     int dev = 0, v = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    dyn_max = v - int(2 * NB2 * GP * sizeof(double)) - 1024;
+    const int m = v - int(2 * NB2 * GP * sizeof(double)) - 1024;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_svd_round<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, m);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_svd_round<false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
-  }
+                              hipFuncAttributeMaxDynamicSharedMemorySize, m);
+    return m;
+  }();
   return dyn_max;
 }
 
@@ -1280,18 +1282,17 @@ size_t gram_eig_lds_bytes(int k) {
 }
 
 size_t jacobi_small_lds_limit() {
-  static size_t limit = 0;
-  if (!limit) {
+  static const size_t limit = [] {
     int dev = 0;
     int v = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    limit = size_t(v);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jacobi_small),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit) - 64);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v - 64);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram_eig),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit) - 64);
-  }
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v - 64);
+    return size_t(v);
+  }();
   return limit;
 }
 
@@ -1598,15 +1599,14 @@ void truncate_select(hipStream_t st, double* base, const int64_t* s_offs, const 
   int64_t n2 = 1;
   while (n2 < total) n2 <<= 1;
   const size_t lds = size_t(n2) * sizeof(double);
-  static size_t limit = 0;
-  if (!limit) {
+  static const size_t limit = [] {
     int dev = 0, v = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    limit = size_t(v) - 256;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_truncate<true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit));
-  }
+                              hipFuncAttributeMaxDynamicSharedMemorySize, v - 256);
+    return size_t(v) - 256;
+  }();
   if (lds <= limit)
     hipLaunchKernelGGL(k_truncate<true>, dim3(1), dim3(1024), lds, st, base, s_offs, ks, nsec, scratch_off, total,
                        tol, pw, min_size, max_size, kept, thr_out);

@@ -111,15 +111,15 @@ __global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __re
 
 void gram_check(hipStream_t st, const double* base, const void* descs, int n, int* flags, double tol, int kmax) {
   if (n <= 0) return;
-  static size_t limit = 0;
-  if (!limit) {
+  static const size_t limit = [] {
     int dev = 0, v = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    limit = size_t(v) - kLargeThreads * sizeof(double) - 64;
+    const size_t l = size_t(v) - kLargeThreads * sizeof(double) - 64;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gram_check),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit));
-  }
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(l));
+    return l;
+  }();
   const size_t lds = size_t(kmax) * sizeof(double);
   if (lds > limit) throw std::runtime_error("gram check: sector wider than the LDS holds");
   hipLaunchKernelGGL(k_gram_check, dim3(n), dim3(kLargeThreads), lds, st, base,
@@ -269,15 +269,15 @@ void pchol_panel(hipStream_t st, double* base, const void* descs, int n, int j0,
 }
 
 size_t pchol_lds_limit() {
-  static size_t limit = 0;
-  if (!limit) {
+  static const size_t limit = [] {
     int dev = 0, v = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) v = 65536;
-    limit = size_t(v) - 1024;  // static arrays of the kernel
+    const size_t l = size_t(v) - 1024;  // static arrays of the kernel
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pchol_panel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(limit));
-  }
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(l));
+    return l;
+  }();
   return limit;
 }
 

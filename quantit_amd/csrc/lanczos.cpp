@@ -1,0 +1,264 @@
+// k-step Lanczos with full FP64 reorthogonalisation for the two-site update:
+// the production eigensolver beside the reference's one-step form
+// (sources/dmrg.cpp:584-608, which it reduces to for two Krylov vectors).
+//
+// The Krylov vectors live in one flat layout: the blocks of the two-site
+// state, in block order, back to back; a vector is a row of V.  The layout
+// starts from theta's blocks and grows when a matvec output has a block
+// outside it (earlier rows get zeros there).  Per step: one matvec (three
+// tensordots), its output scattered into w, two classical Gram-Schmidt passes
+// against all rows (a skinny product and its transpose each), the norm, and
+// the tridiagonal eigensolve; the host reads three numbers (eigenvalue,
+// residual, beta) to decide whether to stop.
+#include <algorithm>
+#include <cmath>
+
+#include "dmrg.hpp"
+#include "kernels.hpp"
+
+namespace qtt {
+
+BTensor matvec_labelled(const BTensor& x, const BTensor& W, const BTensor& Lenv, const BTensor& Renv,
+                        ContractStats* cs);
+
+namespace {
+
+void add_chunks(std::vector<kern::ChunkTask>& tasks, int desc, int64_t numel) {
+  const int64_t nch = (numel + kern::kChunk - 1) / kern::kChunk;
+  for (int64_t c = 0; c < nch; ++c) tasks.push_back({desc, int32_t(c)});
+}
+
+struct Layout {
+  int rank = 0;
+  std::vector<BIdx> idx;  // sorted
+  std::vector<int64_t> off, numel;
+  int64_t n = 0;
+  int find(const BIdx& b) const {
+    auto it = std::lower_bound(idx.begin(), idx.end(), b,
+                               [this](const BIdx& x, const BIdx& y) { return bidx_less(x, y, rank); });
+    if (it != idx.end() && bidx_eq(*it, b, rank)) return int(it - idx.begin());
+    return -1;
+  }
+  void add(const BIdx& b, int64_t ne) {
+    idx.push_back(b);
+    numel.push_back(ne);
+  }
+  void finish() {
+    std::vector<int> ord(idx.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = int(i);
+    std::sort(ord.begin(), ord.end(), [this](int x, int y) { return bidx_less(idx[x], idx[y], rank); });
+    std::vector<BIdx> i2;
+    std::vector<int64_t> n2;
+    for (int o : ord) {
+      i2.push_back(idx[o]);
+      n2.push_back(numel[o]);
+    }
+    idx.swap(i2);
+    numel.swap(n2);
+    off.assign(idx.size(), 0);
+    n = 0;
+    for (size_t i = 0; i < idx.size(); ++i) {
+      off[i] = n;
+      n += numel[i];
+    }
+  }
+};
+
+void launch_copies(Context* ctx, double* ab, const std::vector<kern::Copy2D>& cp,
+                   const std::vector<kern::ChunkTask>& tasks, const char* name) {
+  if (tasks.empty()) return;
+  KernelTimer kt(*ctx, name);
+  auto [dc, dt] = ctx->upload_pack(cp, tasks);
+  kern::copy2d(ctx->stream(), ab, dc, dt, int(tasks.size()));
+}
+
+kern::Copy2D flat_copy(int64_t src, int64_t dst, int64_t ne) {
+  kern::Copy2D c{};
+  c.src_off = src;  // < 0: fill with alpha
+  c.dst_off = dst;
+  c.rows = 1;
+  c.cols = int32_t(ne);
+  c.src_cs = 1;
+  c.dst_cs = 1;
+  c.alpha = src < 0 ? 0.0 : 1.0;
+  return c;
+}
+
+}  // namespace
+
+LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W, const BTensor& Lenv, const BTensor& Renv,
+                             int max_krylov, double tol, ContractStats* cs) {
+  Context* ctx = theta.ctx;
+  double* ab = reinterpret_cast<double*>(ctx->arena().base());
+  const int K = max_krylov;
+  if (K < 2 || K > kern::kMaxKrylov)
+    throw InvalidArgument("Krylov dimension must lie in [2, " + std::to_string(kern::kMaxKrylov) + "]");
+  Layout lay;
+  lay.rank = theta.rank();
+  for (auto& b : theta.blocks)
+    if (b.numel) lay.add(b.idx, b.numel);
+  lay.finish();
+  std::vector<char> present(lay.idx.size(), 1);  // blocks some Krylov vector so far has
+  if (lay.n == 0) throw DomainError("two-site state without elements");
+
+  // scalars: c1[K], c2[K], nrm2, T[2K], out[3 + K]
+  BufPtr sc = ctx->alloc(size_t(6 * K + 8) * sizeof(double));
+  const int64_t s0 = sc->dptr() - ab;
+  const int64_t c1 = s0, c2 = s0 + K, nrm2 = s0 + 2 * K, t_off = s0 + 2 * K + 1, out = s0 + 4 * K + 1;
+  BufPtr flag = ctx->alloc(sizeof(int32_t));
+  kern::fill_i32(ctx->stream(), reinterpret_cast<int32_t*>(flag->cptr()), 1, 0);
+  BufPtr V, w, part;
+  auto alloc_vectors = [&](int64_t n) {
+    V = ctx->alloc(size_t(K) * size_t(n) * sizeof(double));
+    w = ctx->alloc(size_t(n) * sizeof(double));
+    part = ctx->alloc(size_t(K) * size_t((n + kern::kKrylovChunk - 1) / kern::kKrylovChunk + 1) * sizeof(double));
+  };
+  alloc_vectors(lay.n);
+  auto voff = [&](int row) { return V->dptr() - ab + int64_t(row) * lay.n; };
+  auto woff = [&] { return w->dptr() - ab; };
+  auto poff = [&] { return part->dptr() - ab; };
+
+  // v0 = theta / |theta|
+  {
+    std::vector<kern::Copy2D> cp;
+    std::vector<kern::ChunkTask> tasks;
+    for (auto& b : theta.blocks) {
+      if (!b.numel) continue;
+      const int k = lay.find(b.idx);
+      add_chunks(tasks, int(cp.size()), b.numel);
+      cp.push_back(flat_copy(theta.ptr(b) - ab, voff(0) + lay.off[k], b.numel));
+    }
+    launch_copies(ctx, ab, cp, tasks, "lanczos_scatter");
+    KernelTimer kt(*ctx, "lanczos_vec");
+    kern::krylov_dot(ctx->stream(), ab, voff(0), lay.n, 1, voff(0), lay.n, poff(), nrm2);
+    kern::krylov_axpy(ctx->stream(), ab, voff(0), lay.n, 0, nrm2, 0.0, voff(0), voff(0), lay.n, nrm2);
+  }
+
+  LanczosResult res;
+  int m = 0;
+  double host_out[3] = {0, 0, 0};
+  for (int j = 0; j < K; ++j) {
+    // H v_j through a view of row j
+    BTensor x(ctx, theta.shape);
+    x.storage = V;
+    for (size_t b = 0; b < lay.idx.size(); ++b)
+      x.blocks.push_back({lay.idx[b], int64_t(j) * lay.n + lay.off[b], lay.numel[b]});
+    BTensor o = matvec_labelled(x, W, Lenv, Renv, cs);
+    // grow the layout by the output's new blocks
+    std::vector<int> where(o.blocks.size());
+    bool grow = false;
+    for (size_t q = 0; q < o.blocks.size(); ++q) {
+      where[q] = o.blocks[q].numel ? lay.find(o.blocks[q].idx) : 0;
+      grow |= where[q] < 0;
+    }
+    if (grow) {
+      Layout nl;
+      nl.rank = lay.rank;
+      nl.idx = lay.idx;
+      nl.numel = lay.numel;
+      for (size_t q = 0; q < o.blocks.size(); ++q)
+        if (where[q] < 0) nl.add(o.blocks[q].idx, o.blocks[q].numel);
+      nl.finish();
+      BufPtr oldV = V;
+      const int64_t old_n = lay.n;
+      const int64_t oldv = oldV->dptr() - ab;
+      alloc_vectors(nl.n);
+      std::vector<kern::Copy2D> cp;
+      std::vector<kern::ChunkTask> tasks;
+      std::vector<char> np(nl.idx.size(), 0);
+      for (size_t b = 0; b < nl.idx.size(); ++b) {
+        const int k = lay.find(nl.idx[b]);
+        if (k >= 0) np[b] = present[k];
+        for (int r = 0; r <= j; ++r) {
+          add_chunks(tasks, int(cp.size()), nl.numel[b]);
+          cp.push_back(flat_copy(k >= 0 ? oldv + int64_t(r) * old_n + lay.off[k] : -1,
+                                 V->dptr() - ab + int64_t(r) * nl.n + nl.off[b], nl.numel[b]));
+        }
+      }
+      launch_copies(ctx, ab, cp, tasks, "lanczos_regrow");
+      lay = nl;
+      present.swap(np);
+      for (size_t q = 0; q < o.blocks.size(); ++q) where[q] = o.blocks[q].numel ? lay.find(o.blocks[q].idx) : 0;
+    }
+    // w = H v_j in the layout (absent blocks zero)
+    {
+      std::vector<kern::Copy2D> cp;
+      std::vector<kern::ChunkTask> tasks;
+      std::vector<char> hit(lay.idx.size(), 0);
+      for (size_t q = 0; q < o.blocks.size(); ++q) {
+        if (!o.blocks[q].numel) continue;
+        const int k = where[q];
+        hit[k] = 1;
+        add_chunks(tasks, int(cp.size()), lay.numel[k]);
+        cp.push_back(flat_copy(o.ptr(o.blocks[q]) - ab, woff() + lay.off[k], lay.numel[k]));
+      }
+      for (size_t k = 0; k < lay.idx.size(); ++k)
+        if (!hit[k]) {
+          add_chunks(tasks, int(cp.size()), lay.numel[k]);
+          cp.push_back(flat_copy(-1, woff() + lay.off[k], lay.numel[k]));
+        }
+      launch_copies(ctx, ab, cp, tasks, "lanczos_scatter");
+    }
+    {
+      // two classical Gram-Schmidt passes against v_0..v_j, then |w|^2 and
+      // the tridiagonal eigensolve (alpha_j = <v_j|H v_j> with its correction)
+      KernelTimer kt(*ctx, "lanczos_vec");
+      const int mj = j + 1;
+      kern::krylov_dot(ctx->stream(), ab, voff(0), lay.n, mj, woff(), lay.n, poff(), c1);
+      kern::krylov_axpy(ctx->stream(), ab, voff(0), lay.n, mj, c1, -1.0, woff(), woff(), lay.n, -1);
+      kern::krylov_dot(ctx->stream(), ab, voff(0), lay.n, mj, woff(), lay.n, poff(), c2);
+      kern::krylov_axpy(ctx->stream(), ab, voff(0), lay.n, mj, c2, -1.0, woff(), woff(), lay.n, -1);
+      kern::krylov_dot(ctx->stream(), ab, woff(), lay.n, 1, woff(), lay.n, poff(), nrm2);
+    }
+    {
+      KernelTimer kt(*ctx, "lanczos_tridiag");
+      kern::lanczos_tridiag(ctx->stream(), ab, t_off, K, j, c1, c2, nrm2, out,
+                            reinterpret_cast<int*>(flag->cptr()));
+    }
+    int32_t hflag = 0;
+    ctx->read_small(host_out, ab + out, sizeof(host_out));
+    ctx->read_small(&hflag, flag->cptr(), sizeof(int32_t));
+    if (hflag) throw LogicError("nan found in output tensor");
+    const double lam = host_out[0], resid = host_out[1], beta = host_out[2];
+    m = j + 1;
+    const bool done = resid <= tol * std::fabs(lam) || !(beta > 1e-15 * std::fabs(lam)) || m == K;
+    if (done) break;
+    for (size_t q = 0; q < o.blocks.size(); ++q)
+      if (o.blocks[q].numel) present[where[q]] = 1;
+    KernelTimer kt(*ctx, "lanczos_vec");
+    kern::krylov_axpy(ctx->stream(), ab, voff(0), lay.n, 0, nrm2, 0.0, woff(), voff(j + 1), lay.n, nrm2);
+  }
+  res.iterations = m;
+  res.energy = host_out[0];
+  res.residual = host_out[1];
+  // theta' = sum_i y_i v_i, restricted to the blocks the Krylov vectors have
+  {
+    KernelTimer kt(*ctx, "lanczos_vec");
+    kern::krylov_axpy(ctx->stream(), ab, voff(0), lay.n, m, out + 3, 1.0, -1, woff(), lay.n, -1);
+  }
+  BTensor t2(ctx, theta.shape);
+  std::vector<BIdx> keep;
+  for (size_t k = 0; k < lay.idx.size(); ++k)
+    if (present[k]) keep.push_back(lay.idx[k]);
+  t2.allocate(keep);
+  {
+    std::vector<kern::Copy2D> cp;
+    std::vector<kern::ChunkTask> tasks;
+    for (auto& b : t2.blocks) {
+      const int k = lay.find(b.idx);
+      add_chunks(tasks, int(cp.size()), b.numel);
+      cp.push_back(flat_copy(woff() + lay.off[k], t2.ptr(b) - ab, b.numel));
+    }
+    launch_copies(ctx, ab, cp, tasks, "lanczos_scatter");
+  }
+  res.theta = std::move(t2);
+  Shape rank0;
+  rank0.g = theta.shape.g;
+  rank0.sel = q_neutral();
+  res.E = BTensor(ctx, rank0);
+  res.E.storage = sc;
+  res.E.blocks.push_back({BIdx{}, out - (sc->dptr() - ab), 1});
+  return res;
+}
+
+}  // namespace qtt

@@ -80,6 +80,8 @@ struct Batched {
   BufPtr work, outbuf;
   std::vector<int64_t> u_off, v_off, s_off;  // element offsets into outbuf
   int sweeps = 0;
+  int unconverged = 0;  // sectors still rotating when a sweep cap was reached
+  std::vector<char> wide;  // sectors whose rotations ran as multi-work-group rounds (k_svd_round)
 };
 
 // A grouped GEMM whose descriptors are on the device (one staging copy), so
@@ -128,7 +130,8 @@ int env_int(const char* name, int dflt) {
 // Cyclic block Jacobi sweeps over the sectors listed in `which` until a sweep
 // rotates nothing (or max_sweeps).  first_rot[i] receives the number of
 // rotating work-groups of the first sweep of sector which[i].  Returns the
-// number of sweeps run.
+// number of sweeps run; *capped (if given) is raised by the number of
+// sectors still rotating when max_sweeps ran out.
 //
 // The sweeps are pipelined one deep while they rotate much: sweep s+1 is
 // enqueued before the host waits for the rotation counts of sweep s, so the
@@ -139,7 +142,7 @@ int env_int(const char* name, int dflt) {
 // pair-rounds the end is near and the next sweep is enqueued only after the
 // read (a speculative sweep after the last would be launches of no-ops).
 int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& secs, const std::vector<int>& which,
-                  double tol, int inner, int max_sweeps, std::vector<int>* first_rot) {
+                  double tol, int inner, int max_sweeps, std::vector<int>* first_rot, int* capped) {
   const int ns = int(secs.size());
   if (which.empty() || max_sweeps <= 0) return 0;
   BufPtr cnt = ctx->alloc(size_t(2 * ns) * sizeof(int32_t));
@@ -202,6 +205,8 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     speculate = 4 * rotated >= slot_rounds[size_t(sweep & 1)];
     if (queued == sweep + 1 && queued < max_sweeps && enqueue(queued)) ++queued;
   }
+  if (capped && sweeps >= max_sweeps)
+    for (int i = 0; i < ns; ++i) *capped += active[i];
   return sweeps;
 }
 
@@ -337,6 +342,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
   auto lds_bytes = [](int64_t ell, int64_t k) { return size_t(ell * (k + (k & 1))) * sizeof(double); };
   const bool use_small = env_int("QTT_SVD_SMALL", 1) != 0;
   std::vector<char> done(ns, 0), big_only(ns, 0);
+  B.wide.assign(ns, 0);
   std::vector<BufPtr> keep;  // Gram and ping-pong buffers live until the finalize is enqueued
 
   // ---- Gram iterations ----------------------------------------------------------
@@ -360,6 +366,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
                env_int("QTT_SVD_LARGE_GRAM", 1)) {
       wide.push_back(i);
       large[i] = 1;
+      B.wide[i] = 1;
     }
   }
   if (!wide.empty()) {
@@ -594,7 +601,7 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
         gsec.push_back(g);
         vd.push_back({gb + lo[q], gb + gwo[q], gb + sco[q], s.k, s.k, s.k, 0});
       }
-      jacobi_sweeps(ctx, ab, gsec, all, tol, inner, max_sweeps, nullptr);
+      jacobi_sweeps(ctx, ab, gsec, all, tol, inner, max_sweeps, nullptr, &B.unconverged);
       int kv = 0;
       for (auto& v : vd) kv = std::max(kv, v.k);
       KernelTimer kt(*ctx, "svd_vectors");
@@ -715,7 +722,8 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     kern::jacobi_small(ctx->stream(), ab, ctx->upload_vec(jd), int(jd.size()), nullptr, tol, max_sweeps,
                        small_need);
   }
-  B.sweeps += jacobi_sweeps(ctx, ab, B.secs, rest_big, tol, inner, max_sweeps, nullptr);
+  for (int i : rest_big) B.wide[i] = 1;
+  B.sweeps += jacobi_sweeps(ctx, ab, B.secs, rest_big, tol, inner, max_sweeps, nullptr, &B.unconverged);
   // ---- finalize --------------------------------------------------------------
   std::vector<int4> wt;
   for (int i = 0; i < ns; ++i) {
@@ -800,6 +808,13 @@ SvdResult svd(const BTensor& t, size_t split, SvdStats* stats) {
   };
   srt(U);
   srt(V);
+  for (size_t i = 0; i < B.secs.size(); ++i) {
+    const auto& s = B.secs[i];
+    const double m = std::max(s.m, s.n), n = std::min(s.m, s.n);
+    const double f = 14.0 * m * n * n + 8.0 * n * n * n;
+    ctx->svd_model_flops += f;
+    if (i < B.wide.size() && B.wide[i]) ctx->svd_model_flops_wide += f;
+  }
   if (stats) {
     for (auto& s : B.secs) {
       const double m = std::max(s.m, s.n), n = std::min(s.m, s.n);
@@ -808,7 +823,11 @@ SvdResult svd(const BTensor& t, size_t split, SvdStats* stats) {
     }
     stats->sectors += ns;
     stats->sweeps = std::max(stats->sweeps, B.sweeps);
+    stats->unconverged += B.unconverged;
   }
+  ctx->svd_calls += 1;
+  ctx->svd_max_sweeps = std::max<int64_t>(ctx->svd_max_sweeps, B.sweeps);
+  ctx->svd_unconverged += B.unconverged;
   // reshape_as back to the original legs
   std::vector<int64_t> ushape(t.rank(), -1), vshape(t.rank(), -1);
   for (int i = 0; i < t.rank(); ++i) (i < int(split) ? vshape : ushape)[i] = 0;
@@ -832,6 +851,9 @@ EighResult eigh(const BTensor& t, size_t split) {
   const Groups& g = rs.g;
   auto plans = compact_dense_plan(rt);
   Batched B = run_batched(rt, plans, true);
+  ctx->svd_calls += 1;
+  ctx->svd_max_sweeps = std::max<int64_t>(ctx->svd_max_sweeps, B.sweeps);
+  ctx->svd_unconverged += B.unconverged;
   const int ns = int(plans.size());
   std::vector<int64_t> dsz(ns);
   std::vector<Charge> rd_c(ns);

@@ -1,5 +1,7 @@
 #include "runtime.hpp"
 
+#include "kernels.hpp"
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -76,7 +78,7 @@ Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(d
   QTT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   QTT_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
   QTT_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
-  arena_ = std::make_unique<Arena>(arena_bytes);
+  arena_ = std::make_shared<Arena>(arena_bytes);
   stage_cap_ = round_up(stage_bytes);
   QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_cap_, hipHostMallocDefault));
   dev_stage_ = alloc(stage_cap_);
@@ -104,7 +106,8 @@ Context::~Context() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-void* Context::upload(const void* data, size_t bytes) {
+// Reserves `bytes` of the staging ring at *pos and fills its host half.
+char* Context::stage_host(const void* data, size_t bytes, size_t* pos) {
   size_t need = round_up(bytes ? bytes : 1);
   if (need > stage_cap_) throw OutOfMemory("descriptor upload larger than the staging ring");
   if (stage_pos_ + need > stage_cap_) {
@@ -114,15 +117,21 @@ void* Context::upload(const void* data, size_t bytes) {
     if (side_) QTT_HIP(hipStreamSynchronize(side_));
     stage_pos_ = 0;
   }
+  *pos = stage_pos_;
   char* h = host_stage_ + stage_pos_;
-  char* d = dev_stage_->cptr() + stage_pos_;
-  if (bytes) {
-    std::memcpy(h, data, bytes);
-    QTT_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
-  }
+  if (bytes) std::memcpy(h, data, bytes);
   stage_pos_ += need;
+  return h;
+}
+
+void* Context::upload(const void* data, size_t bytes) {
+  size_t pos = 0;
+  char* h = stage_host(data, bytes, &pos);
+  char* d = dev_stage_->cptr() + pos;
+  if (bytes) QTT_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
   return d;
 }
+
 
 void Context::fork() {
   if (!side_) QTT_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));

@@ -41,11 +41,15 @@ class Arena {
   std::mutex mu_;
 };
 
+// A region of an arena.  It shares ownership of the arena, so a tensor that
+// outlives its context still releases its bytes into live memory (the arena
+// is freed with its last region).
 struct DeviceBuffer {
-  Arena* arena;
+  std::shared_ptr<Arena> arena;
   size_t off;
   size_t bytes;
-  DeviceBuffer(Arena* a, size_t nbytes) : arena(a), off(nbytes ? a->alloc(nbytes) : 0), bytes(nbytes) {}
+  DeviceBuffer(std::shared_ptr<Arena> a, size_t nbytes)
+      : arena(std::move(a)), off(nbytes ? arena->alloc(nbytes) : 0), bytes(nbytes) {}
   ~DeviceBuffer() {
     if (bytes) arena->release(off);
   }
@@ -79,7 +83,7 @@ class Context {
   void fork();
   void join();
   Arena& arena() { return *arena_; }
-  BufPtr alloc(size_t bytes) { return std::make_shared<DeviceBuffer>(arena_.get(), bytes); }
+  BufPtr alloc(size_t bytes) { return std::make_shared<DeviceBuffer>(arena_, bytes); }
 
   // Copies host descriptors to device memory in stream order; returns a device
   // pointer valid for kernels enqueued afterwards on this context's stream.
@@ -128,7 +132,23 @@ class Context {
   int32_t* read_slot(int slot) const { return host_read_ + size_t(slot) * read_cap_; }
   hipEvent_t read_event(int slot) const { return read_ev_[slot]; }
 
+  // Handle object of the C ABI that owns this context (opaque here).
+  void* owner = nullptr;
+  // Block SVD health over the context's life: calls, the largest number of
+  // sweeps/Gram iterations one call needed, and sectors whose Jacobi sweeps
+  // reached the sweep cap still rotating.
+  int64_t svd_calls = 0, svd_max_sweeps = 0, svd_unconverged = 0;
+  // SVD model FLOPs (sum over sectors of 14 m n^2 + 8 n^3, m >= n) of every
+  // block SVD, and of the sectors whose rotations ran as multi-work-group
+  // rounds
+  double svd_model_flops = 0, svd_model_flops_wide = 0;
   Precision precision = Precision::FP64;
+  // Two-site eigensolver of DMRG: 0 = the reference's one-step Lanczos;
+  // k >= 2 = k-step Lanczos with full reorthogonalisation, residual tolerance
+  // lanczos_tol (relative to the eigenvalue).
+  int lanczos_krylov = 0;
+  double lanczos_tol = 1e-12;
+  int64_t lanczos_updates = 0, lanczos_vectors = 0;  // k-step mode: updates and Krylov vectors built
   // Reference-shaped baseline: one GEMM launch per block pair (mm, addmm_).
   bool per_block_launch = false;
   // Independent work of one call may run on the side stream.  Processes that
@@ -151,7 +171,7 @@ class Context {
   hipStream_t stream_ = nullptr;
   hipStream_t side_ = nullptr;
   hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
-  std::unique_ptr<Arena> arena_;
+  std::shared_ptr<Arena> arena_;
   char* host_stage_ = nullptr;
   BufPtr dev_stage_;
   size_t stage_cap_ = 0, stage_pos_ = 0;
@@ -168,6 +188,7 @@ class Context {
   size_t read_cap_ = 0;  // words per slot
   hipEvent_t read_ev_[2] = {nullptr, nullptr};
   hipEvent_t sync_ev_ = nullptr;
+  char* stage_host(const void* data, size_t bytes, size_t* pos);
 };
 
 // RAII helper: brackets one launch with events when profiling is enabled.
@@ -198,7 +219,7 @@ struct HostTimer {
   const char* name;
   bool on;
   std::chrono::steady_clock::time_point t0;
-  HostTimer(Context& c, const char* n) : ctx(c), name(n), on(c.profiling) {
+  HostTimer(Context& c, const char* n) : ctx(c), name(n), on(c.profiling && n != nullptr) {
     if (on) t0 = std::chrono::steady_clock::now();
   }
   void stop() {

@@ -4,7 +4,10 @@
 #include <algorithm>
 #include <climits>
 #include <cstdlib>
+#include <cstring>
+#include <memory>
 #include <numeric>
+#include <tuple>
 
 #include "btensor.hpp"
 #include "kernels.hpp"
@@ -67,12 +70,79 @@ struct KeyEnt {
   int src;
 };
 
+// Where one contraction keeps its temporaries (packed operands, split-K
+// partials) and its launch descriptors.
+class TdWork {
+ public:
+  virtual ~TdWork() = default;
+  // element offset (from the arena base) of `bytes` of device scratch
+  virtual int64_t take(size_t bytes) = 0;
+  // device copy of host bytes, ordered on the context stream before the
+  // work enqueued afterwards
+  virtual void* put(const void* src, size_t bytes) = 0;
+  virtual bool timed() const = 0;
+};
+
+// Temporaries from the context arena, descriptors through its staging ring.
+class ArenaWork : public TdWork {
+ public:
+  explicit ArenaWork(Context* c) : ctx_(c) {}
+  int64_t take(size_t bytes) override {
+    keep_.push_back(ctx_->alloc(bytes));
+    return keep_.back()->dptr() - reinterpret_cast<double*>(ctx_->arena().base());
+  }
+  void* put(const void* src, size_t bytes) override { return ctx_->upload(src, bytes); }
+  bool timed() const override { return true; }
+
+ private:
+  Context* ctx_;
+  std::vector<BufPtr> keep_;  // released at the end of the call: later work is stream-ordered after it
+};
+
+// Several descriptor lists in one copy; each list starts 16-byte aligned.
+template <class... T>
+std::tuple<T*...> put_pack(TdWork& w, const std::vector<T>&... v) {
+  constexpr size_t kA = 16;
+  const size_t sizes[] = {v.size() * sizeof(T)...};
+  size_t offs[sizeof...(T)], total = 0;
+  for (size_t i = 0; i < sizeof...(T); ++i) {
+    offs[i] = total;
+    total += (sizes[i] + kA - 1) / kA * kA;
+  }
+  std::vector<char> buf(total ? total : 1);
+  const void* src[] = {static_cast<const void*>(v.data())...};
+  for (size_t i = 0; i < sizeof...(T); ++i)
+    if (sizes[i]) std::memcpy(buf.data() + offs[i], src[i], sizes[i]);
+  char* d = static_cast<char*>(w.put(buf.data(), total));
+  size_t i = 0;
+  return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
+}
+
+// A kernel timer that records only when the work allows it.
+struct MaybeTimer {
+  std::unique_ptr<KernelTimer> t;
+  MaybeTimer(Context& c, const char* name, const TdWork& w) {
+    if (w.timed()) t = std::make_unique<KernelTimer>(c, name);
+  }
+};
+
+BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
+                       const std::vector<int64_t>& db_in, ContractStats* stats, TdWork& work, const BTensor* into);
+
 }  // namespace
 
-BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
-                  const std::vector<int64_t>& db_in, ContractStats* stats) {
+BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da, const std::vector<int64_t>& db,
+                  ContractStats* stats) {
+  ArenaWork w(a.ctx);
+  return tensordot_impl(a, b, da, db, stats, w, nullptr);
+}
+
+namespace {
+
+BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
+                       const std::vector<int64_t>& db_in, ContractStats* stats, TdWork& work, const BTensor* into) {
   Context* ctx = a.ctx;
-  HostTimer ht_match(*ctx, "host_td_match");
+  HostTimer ht_match(*ctx, work.timed() ? "host_td_match" : nullptr);
   if (a.shape.g != b.shape.g) throw CheckError("the two tensors have different type of conserved quantities");
   if (da_in.size() != db_in.size()) throw CheckError("both dimension lists should have the same length.");
   const int ra = a.rank(), rb = b.rank();
@@ -204,12 +274,21 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   oidx.reserve(outs.size());
   for (auto& o : outs) oidx.push_back(o.idx);
   ht_match.stop();
-  {
+  if (into) {
+    // the caller's output: the structure this contraction produces, blocks
+    // in the same order
+    const Shape& is = into->shape;
+    bool same = is.g == os.g && is.secs == os.secs && is.cvals == os.cvals && is.sel == os.sel &&
+                into->blocks.size() == oidx.size();
+    for (size_t k = 0; same && k < oidx.size(); ++k) same = bidx_eq(into->blocks[k].idx, oidx[k], na + nb);
+    if (!same) throw InvalidArgument("output tensor does not have the structure of the contraction result");
+    out = *into;
+  } else {
     HostTimer ht(*ctx, "host_td_alloc");
     out.allocate(oidx);
   }
   if (outs.empty()) return out;
-  HostTimer ht_plan(*ctx, "host_td_plan");
+  HostTimer ht_plan(*ctx, work.timed() ? "host_td_plan" : nullptr);
 
   // matrix views of the operand blocks; blocks whose permuted view is not a
   // plain strided matrix are packed (row-major) into a scratch buffer
@@ -278,18 +357,16 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     if (used_a[k]) plan_view(a, a.blocks[k], free_a, ca, va[k]);
   for (size_t k = 0; k < b.blocks.size(); ++k)
     if (used_b[k]) plan_view(b, b.blocks[k], cb, free_b, vb[k]);
-  BufPtr scratch;
   if (pack_elems) {
-    scratch = ctx->alloc(size_t(pack_elems) * sizeof(double));
-    const int64_t soff = scratch->dptr() - ab;
+    const int64_t soff = work.take(size_t(pack_elems) * sizeof(double));
     for (auto& d : pdesc) d.dst_off += soff;
     for (auto& v : va)
       if (v.need_pack) v.off += soff;
     for (auto& v : vb)
       if (v.need_pack) v.off += soff;
-    KernelTimer kt(*ctx, "pack_permute");
-    HostTimer ht(*ctx, "host_td_upload");
-    auto [dd, dt] = ctx->upload_pack(pdesc, ptasks);
+    MaybeTimer kt(*ctx, "pack_permute", work);
+    HostTimer ht(*ctx, work.timed() ? "host_td_upload" : nullptr);
+    auto [dd, dt] = put_pack(work, pdesc, ptasks);
     ht.stop();
     kern::permute_copy(ctx->stream(), ab, dd, dt, int(ptasks.size()));
   }
@@ -342,8 +419,8 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   // mode); otherwise every block takes MFMA tiles, so a contraction is one
   // launch either way.
   const char* rows_env = std::getenv("QTT_GEMM_ROWS");  // "0": MFMA tiles for every block
-  bool all_narrow = ctx->precision == Precision::FP64 && !ctx->per_block_launch &&
-                    !(rows_env && std::atoi(rows_env) == 0);
+  const bool per_block = ctx->per_block_launch && !into;
+  bool all_narrow = ctx->precision == Precision::FP64 && !per_block && !(rows_env && std::atoi(rows_env) == 0);
   for (size_t o = 0; o < outs.size() && all_narrow; ++o)
     all_narrow = gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK;
   std::vector<char> narrow(outs.size(), char(all_narrow));
@@ -364,12 +441,11 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
   std::vector<char> split_block(outs.size(), 0);
   std::vector<kern::SumParts> sums;
   std::vector<kern::ChunkTask> sum_tasks;
-  BufPtr partials;
   {
     constexpr int64_t kTargetTiles = 2048, kSplitMinK = 64;
     constexpr int64_t kMaxSplit = 8;
     const char* split_env = std::getenv("QTT_GEMM_SPLIT_K");  // "0": never split
-    const bool split_ok = !ctx->per_block_launch && !(split_env && std::atoi(split_env) == 0);
+    const bool split_ok = !per_block && !(split_env && std::atoi(split_env) == 0);
     const int64_t split = split_ok ? std::min(kMaxSplit, kTargetTiles / std::max<int64_t>(mfma_tiles, 1)) : 1;
     std::vector<int> nparts(outs.size(), 0);
     int64_t part_elems = 0;
@@ -382,8 +458,7 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
         part_elems += np * gouts[o].M * gouts[o].N;
       }
     if (part_elems) {
-      partials = ctx->alloc(size_t(part_elems) * sizeof(double));
-      int64_t pos = partials->dptr() - ab;
+      int64_t pos = work.take(size_t(part_elems) * sizeof(double));
       for (size_t o = 0; o < outs.size(); ++o) {
         const int np = nparts[o];
         if (!np) continue;
@@ -458,7 +533,7 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     stats->pairs += long(gpairs.size());
     stats->tiles += long(tiles.size());
   }
-  if (ctx->per_block_launch) {
+  if (per_block) {
     // reference-shaped baseline: one launch per block pair (mm, then addmm_);
     // the per-pair descriptors and tile lists go up in one staging copy
     std::vector<kern::GemmOut> one;
@@ -490,26 +565,28 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
     return out;
   }
   ht_plan.stop();
-  HostTimer ht_up(*ctx, "host_td_upload");
+  HostTimer ht_up(*ctx, work.timed() ? "host_td_upload" : nullptr);
   auto [d_tiles, d_outs, d_pairs, d_rows, d_sums, d_sum_tasks] =
-      ctx->upload_pack(tiles, launch_outs, gpairs, row_tasks, sums, sum_tasks);
+      put_pack(work, tiles, launch_outs, gpairs, row_tasks, sums, sum_tasks);
   ht_up.stop();
   if (!row_tasks.empty()) {
-    KernelTimer kt(*ctx, "gemm_rows");
+    MaybeTimer kt(*ctx, "gemm_rows", work);
     kern::gemm_rows(ctx->stream(), ab, d_rows, int(row_tasks.size()), d_outs, d_pairs, 1.0, 0.0);
   }
   if (ctx->precision == Precision::SPLIT_FP32) {
-    KernelTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_split");
+    MaybeTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_split", work);
     kern::gemm_split(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
   } else {
-    KernelTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_f64");
+    MaybeTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_f64", work);
     kern::gemm_f64(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
   }
   if (!sums.empty()) {
-    KernelTimer kt(*ctx, "gemm_split_k_sum");
+    MaybeTimer kt(*ctx, "gemm_split_k_sum", work);
     kern::sum_parts(ctx->stream(), ab, d_sums, d_sum_tasks, int(sum_tasks.size()));
   }
   return out;
 }
+
+}  // namespace
 
 }  // namespace qtt
