@@ -736,11 +736,14 @@ def compute_last_index(vd, tol, pow_, min_size, max_size):
 
 def _remove_unit_blocks(U, kd):
     """``remove_unit_blocks`` lambda of ``truncate_impl``
-    (btensor_linalg.cpp:690-709), restated step for step on the ordered block
-    list.  It advances past one matching block per iteration, so a matching
-    block that directly follows another matching block is kept (as in the
-    reference)."""
-    lst = U.sorted_items()
+    (btensor_linalg.cpp:690-709), restated step for step on the block list in
+    its current order (the dict's insertion order stands for the flat_map's
+    vector).  It advances past one matching block per iteration, so a
+    matching block that directly follows another matching block is kept (as
+    in the reference), and its swaps can leave the list unsorted: the next
+    call (the next erased sector) then walks that unsorted order, as the
+    reference's does."""
+    lst = list(U.blocks.items())
     n = len(lst)
     src = dest = 0
     while dest != n:
@@ -765,6 +768,8 @@ def truncate_impl(d, unitaries, max_size, min_size, tol, pow_=2.0):
     the last leg is updated.  d blocks are visited in reverse order."""
     d = d.copy()
     unitaries = [u.copy() for u in unitaries]
+    for u in unitaries:   # the flat_map order the reference starts from
+        u.blocks = dict(u.sorted_items())
     items = d.sorted_items()
     if not items:
         return d, unitaries, None

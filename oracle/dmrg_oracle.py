@@ -14,6 +14,12 @@ Pinning (tests/test_oracle_dmrg.py):
                          cylinders; XX chain free-fermion closed form.
   * cylinder builder  -- <psi|H|psi> equal to the 4x8 fixture MPO
                          (tests/2dHeisenberg/mpotensor_*.txt).
+  * lanczos_ground    -- (this build's k-step eigensolver, no reference
+                         counterpart) with two vectors it equals the one-step
+                         update (eig2x2 of one_step_lanczos); its Ritz value
+                         equals the lowest eigenvalue of the dense two-site
+                         Hamiltonian built column by column from the matvec;
+                         DMRG with it reaches exact-diagonalisation energies.
 The random initial MPS follows the structure of ``random_bMPS``
 (sources/MPT.cpp:311-462) but draws from numpy's PCG64, not from std::mt19937
 and torch's CPU generator, so it is this project's own seeded input.
@@ -22,7 +28,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .btensor_oracle import (BTensor, add, eye_like, mul, ones_like, q_add, q_inv, q_neutral,
+from .btensor_oracle import (BTensor, add, eye_like, full_dot, mul, ones_like, q_add, q_inv, q_neutral,
                              rand_like, scal, shape_product, svd_split, svd_truncated, tensordot)
 
 
@@ -473,6 +479,47 @@ def one_step_lanczos(theta, W12, Lenv, Renv):
     return psi, _scalar(a0t), _scalar(a1t), float(b)
 
 
+def lanczos_ground(theta, W12, Lenv, Renv, max_krylov, tol):
+    """k-step Lanczos with full reorthogonalisation on the two-site matvec
+    (the production eigensolver of this build; the reference has only the
+    one-step form, to which this reduces for max_krylov = 2).
+
+    v0 = theta/|theta|; step j: w = H v_j, two classical Gram-Schmidt passes
+    against v_0..v_j (coefficients c, c'), alpha_j = c_j + c'_j,
+    beta_j = |w|; the lowest eigenpair (E, y) of the (j+1)-square tridiagonal
+    (numpy's dense symmetric eigensolver), y[0] >= 0; stop when
+    beta_j |y_j| <= tol |E|, beta_j <= 1e-15 |E| or j+1 = max_krylov,
+    else v_{j+1} = w / beta_j.  Returns (E, sum_i y_i v_i, vectors used)."""
+    nrm = np.sqrt(full_dot(theta, theta)[0])
+    V = [scal(theta, 1.0 / nrm)]
+    alpha, beta = [], []
+    E, y = None, None
+    for j in range(max_krylov):
+        w = matvec(V[j], W12, Lenv, Renv)
+        c = [full_dot(w, v)[0] for v in V]
+        for ci, v in zip(c, V):
+            w = add(w, v, -ci)
+        c2 = [full_dot(w, v)[0] for v in V]
+        for ci, v in zip(c2, V):
+            w = add(w, v, -ci)
+        alpha.append(c[j] + c2[j])
+        b = float(np.sqrt(full_dot(w, w)[0]))
+        beta.append(b)
+        m = j + 1
+        T = np.diag(alpha) + np.diag(beta[:m - 1], 1) + np.diag(beta[:m - 1], -1)
+        vals, vecs = np.linalg.eigh(T)
+        E, y = float(vals[0]), vecs[:, 0]
+        if y[0] < 0:
+            y = -y
+        if b * abs(y[-1]) <= tol * abs(E) or not (b > 1e-15 * abs(E)) or m == max_krylov:
+            break
+        V.append(scal(w, 1.0 / b))
+    out = scal(V[0], float(y[0]))
+    for yi, v in zip(y[1:], V[1:]):
+        out = add(out, v, float(yi))
+    return E, out, len(y)
+
+
 def two_sites_update(theta, W12, Lenv, Renv):
     """``two_sites_update_impl`` (dmrg.cpp:623-638)."""
     psi, a0, a1, b = one_step_lanczos(theta, W12, Lenv, Renv)
@@ -500,18 +547,26 @@ class DmrgOptions:
     """``dmrg_options`` (include/dmrg_options.h:15-57) defaults."""
 
     def __init__(self, cutoff=1e-6, convergence_criterion=1e-5, maximum_bond=2 ** 63 - 1,
-                 minimum_bond=4, maximum_iterations=1000):
+                 minimum_bond=4, maximum_iterations=1000, max_krylov=0, lanczos_tol=1e-12):
         self.cutoff = cutoff
         self.convergence_criterion = convergence_criterion
         self.maximum_bond = maximum_bond
         self.minimum_bond = minimum_bond
         self.maximum_iterations = maximum_iterations
+        # 0: the reference's one-step Lanczos; >= 2: k-step Lanczos
+        self.max_krylov = max_krylov
+        self.lanczos_tol = lanczos_tol
 
 
 def dmrg_update(mpo, W12, mps, env, oc, step, opts, trace=None):
     """``dmrg_2sites_update::operator()`` (dmrg.cpp:163-206)."""
     theta = tensordot(mps[oc], mps[oc + 1], [2], [0])
-    E, theta2, coeffs = two_sites_update(theta, W12[oc], env[oc - 1], env[oc + 2])
+    if getattr(opts, "max_krylov", 0) >= 2:
+        E, theta2, nvec = lanczos_ground(theta, W12[oc], env[oc - 1], env[oc + 2], opts.max_krylov,
+                                         opts.lanczos_tol)
+        coeffs = (None, None, None)
+    else:
+        E, theta2, coeffs = two_sites_update(theta, W12[oc], env[oc - 1], env[oc + 2])
     u, d, v = svd_truncated(theta2, 2, opts.cutoff, opts.minimum_bond, opts.maximum_bond)
     nrm = np.sqrt(sum(float(np.sum(b * b)) for b in d.blocks.values()))
     d = scal(d, 1.0 / nrm)
@@ -529,17 +584,24 @@ def dmrg_update(mpo, W12, mps, env, oc, step, opts, trace=None):
     return E
 
 
-def dmrg(mpo, mps, opts, log=None, trace=None):
+def dmrg(mpo, mps, opts, log=None, trace=None, oc=0):
     """``dmrg(bMPO&, bMPS&, options, logger)`` + ``details::dmrg_impl``
-    (dmrg.cpp:92-100, 219-273) starting with the orthogonality centre at 0.
-    Returns (E, mps, per-sweep energies)."""
+    (dmrg.cpp:92-100, 219-273) with the orthogonality centre at ``oc``:
+    sweeps start toward the far end (a centre on the last site starts one
+    site to the left, :232-236) and the centre is moved back at the end
+    (:257-266).  Returns (E, mps, per-sweep energies)."""
     L = len(mpo)
     mps = list(mps)
-    oc = 0
     env = generate_env(mpo, mps, oc)
     W12 = two_site_hamil(mpo)
     n_step = len(W12) - 1 + (len(W12) == 1)
-    step = 1
+    init_pos = oc
+    step = 1 if oc == 0 else -1
+    if len(W12) == 1:
+        step = 0
+    if oc == L - 1:
+        init_pos -= 1
+        oc -= 1
     E0 = 100000.0
     energies = []
     for it in range(opts.maximum_iterations):
@@ -555,6 +617,10 @@ def dmrg(mpo, mps, opts, log=None, trace=None):
         Eold, E0 = E0, E
         if not (abs((E0 - Eold) / E0) > opts.convergence_criterion):
             break
+    if oc != init_pos:
+        if oc != init_pos - 1 and init_pos != L - 1:
+            raise RuntimeError(f"the orthogonality center finished somewhere surprising! final oc: {oc}")
+        mps, oc = move_oc(mps, oc, init_pos)
     return E0, mps, energies
 
 

@@ -50,3 +50,19 @@ def test_error_channel_without_gpu(lib):
     st = lib.qtt_btensor_info(None, None, None, None, None, None)
     assert st == 1
     assert b"null pointer" in lib.qtt_last_error()
+
+
+def test_c_client_compiles_against_the_header(tmp_path):
+    """The header is usable from plain C99: the client compiles and links
+    against the library (running it needs a GPU)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    out = tmp_path / "client"
+    res = subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                          os.path.join(ROOT, "tests", "abi_client.c"), "-o", str(out),
+                          "-L", os.path.join(ROOT, "quantit_amd"), "-lquantit_amd", "-lm"],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr

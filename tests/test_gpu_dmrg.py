@@ -80,3 +80,114 @@ def test_dmrg_heisenberg_exact(ctx, L):
     Ec = qa.contract_energy(new, new, dmpo)
     nrm = qa.contract_norm(new, new)
     assert abs(Ec / nrm - e0) <= 1e-9 * abs(e0)
+
+
+@pytest.mark.parametrize("oc", [9, 4])
+def test_dmrg_from_other_centre_matches_oracle(ctx, oc):
+    """qtt_dmrg_from_center against the oracle's dmrg_impl from the same
+    centre (sources/dmrg.cpp:224-236, :257-266): per-sweep energies, the
+    final state's structure bit-exact and its dense vector equal up to sign."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(11)
+    L = 10
+    dense = dor.xxz_field_dense(0.9, rng.uniform(-1.0, 1.0, L))
+    mpo, mps = _setup(dense, seed=6)
+    mps, _ = dor.move_oc(mps, 0, oc)
+    opts = dor.DmrgOptions(cutoff=1e-8, convergence_criterion=0.0, maximum_bond=20, minimum_bond=4,
+                           maximum_iterations=3)
+    E_ref, mps_ref, Es_ref = dor.dmrg(mpo, [t.copy() for t in mps], opts, oc=oc)
+    E, new, log = qa.dmrg([to_dev(ctx, t) for t in mpo], [to_dev(ctx, t) for t in mps],
+                          qa.DmrgOptions(cutoff=1e-8, convergence_criterion=0.0, maximum_bond=20, minimum_bond=4,
+                                         maximum_iterations=3), oc=oc)
+    for a, b in zip(log["energies"], Es_ref):
+        assert abs(a - b) <= 1e-10 * abs(b)
+    out = [from_dev(t) for t in new]
+    for a, b in zip(out, mps_ref):
+        assert a.secs == b.secs and a.cvals == b.cvals and sorted(a.blocks) == sorted(b.blocks)
+    ov = abs(float(_mps_dense(out) @ _mps_dense(mps_ref)))
+    assert abs(ov - 1.0) < 1e-9
+
+
+def _lanczos_ctx(ctx, k, tol):
+    class Mode:
+        def __enter__(self):
+            ctx.set_lanczos(k, tol)
+
+        def __exit__(self, *exc):
+            ctx.set_lanczos(0)
+    return Mode()
+
+
+def test_lanczos_two_vectors_equals_one_step(ctx):
+    """k-step Lanczos with two Krylov vectors is the reference's one-step
+    update: the same per-sweep energies and the same MPS structure as the
+    default mode."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(12)
+    L = 10
+    mpo, mps = _setup(dor.xxz_field_dense(0.8, rng.uniform(-1.0, 1.0, L)), seed=2)
+    opts = qa.DmrgOptions(cutoff=1e-8, convergence_criterion=0.0, maximum_bond=20, minimum_bond=4,
+                          maximum_iterations=3)
+    E1, m1, l1 = qa.dmrg([to_dev(ctx, t) for t in mpo], [to_dev(ctx, t) for t in mps], opts)
+    with _lanczos_ctx(ctx, 2, 0.0):
+        E2, m2, l2 = qa.dmrg([to_dev(ctx, t) for t in mpo], [to_dev(ctx, t) for t in mps], opts)
+    for a, b in zip(l1["energies"], l2["energies"]):
+        assert abs(a - b) <= 1e-12 * abs(a)
+    for a, b in zip(m1, m2):
+        ga, gb = from_dev(a), from_dev(b)
+        assert ga.secs == gb.secs and sorted(ga.blocks) == sorted(gb.blocks)
+
+
+def test_dmrg_lanczos_matches_oracle_and_exact(ctx):
+    """k-step mode (up to 24 vectors, residual 1e-12) against the oracle's
+    restatement on the same inputs, sweep by sweep, and against exact
+    diagonalisation; each update builds more than two vectors."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(13)
+    L = 10
+    dense = dor.xxz_field_dense(0.6, rng.uniform(-1.0, 1.0, L))
+    mpo, mps = _setup(dense, seed=4)
+    o = dict(cutoff=1e-10, convergence_criterion=0.0, maximum_bond=32, minimum_bond=4, maximum_iterations=3)
+    E_ref, _, Es_ref = dor.dmrg(mpo, [t.copy() for t in mps],
+                                dor.DmrgOptions(**o, max_krylov=24, lanczos_tol=1e-12))
+    before = ctx.lanczos_stats()
+    with _lanczos_ctx(ctx, 24, 1e-12):
+        E, new, log = qa.dmrg([to_dev(ctx, t) for t in mpo], [to_dev(ctx, t) for t in mps], qa.DmrgOptions(**o))
+    after = ctx.lanczos_stats()
+    print("device", log["energies"], "oracle", Es_ref, "vectors/update",
+          (after["vectors"] - before["vectors"]) / max(1, after["updates"] - before["updates"]), flush=True)
+    for a, b in zip(log["energies"], Es_ref):
+        assert abs(a - b) <= 1e-10 * abs(b)
+    assert after["vectors"] - before["vectors"] > 2 * (after["updates"] - before["updates"])
+    H = dor.mpo_to_dense_matrix(dense)
+    sz = np.array([bin(i).count("1") for i in range(2 ** L)])
+    sec = np.where(sz == L // 2)[0]
+    e0 = np.linalg.eigvalsh(H[np.ix_(sec, sec)])[0]
+    assert abs(E - e0) <= 1e-10 * abs(e0)
+
+
+def test_lanczos_chain_L32_D128_converges(ctx):
+    """The L=32, D=128 chain in k-step mode reaches |dE/E| <= 1e-10 between
+    sweeps within 6 sweeps, below the one-step trajectory's sixth-sweep
+    energy."""
+    import quantit_amd as qa
+    from quantit_amd import models
+    L = 32
+    mpo_parts = models.bmpo_parts(models.spin_mpo(L, models.chain_bonds(L)))
+    mps_parts = models.random_bmps_parts(L, 4, q=0, seed=0)
+    opts = qa.DmrgOptions(cutoff=1e-10, convergence_criterion=1e-10, maximum_bond=128, minimum_bond=4,
+                          maximum_iterations=6)
+
+    def run():
+        mpo = [qa.BTensor.from_parts(ctx, **p) for p in mpo_parts]
+        mps = qa.mps_prepare([qa.BTensor.from_parts(ctx, **p) for p in mps_parts])
+        return qa.dmrg(mpo, mps, opts)
+
+    E1, _, l1 = run()
+    with _lanczos_ctx(ctx, 32, 1e-12):
+        E2, _, l2 = run()
+    es = l2["energies"]
+    print("one-step", l1["energies"], "k-step", es, flush=True)
+    assert len(es) < 6 or abs((es[-1] - es[-2]) / es[-1]) <= 1e-10
+    assert abs((es[-1] - es[-2]) / es[-1]) <= 1e-10
+    assert E2 <= E1 + 1e-12 * abs(E1)

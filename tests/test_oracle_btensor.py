@@ -238,3 +238,57 @@ def test_eigh_equals_dense():
     Ud = U.to_dense()
     dd = np.concatenate([d.blocks[(i,)] for i in range(len(d.secs[0]))])
     assert np.allclose(A.to_dense() @ Ud, Ud * dd, atol=1e-12)
+
+
+def test_c5_contraction_block_set_known_answer():
+    """include/blockTensor/btensor.h:1585-1608: C = A.tensordot(B, {1}, {1})
+    with C<5> charges has exactly the blocks (0,1,0), (1,0,0), (1,1,1), each
+    the dense tensordot of one block pair; the full self-contraction of A is
+    the sum of its blocks' squared norms."""
+    from oracle.btensor_oracle import tensordot as td
+    rng = np.random.default_rng(0)
+    C5 = (5,)
+    A = BTensor.from_sections(C5, [[(2, (0,)), (3, (1,))], [(2, (0,)), (3, (4,))]], (0,))
+    A.blocks[(0, 0)] = rng.uniform(size=(2, 2))
+    A.blocks[(1, 1)] = rng.uniform(size=(3, 3))
+    B = BTensor.from_sections(C5, [[(3, (4,)), (2, (0,))], [(2, (0,)), (3, (1,))], [(1, (1,)), (3, (0,))]], (1,))
+    B.blocks[(1, 0, 0)] = rng.uniform(size=(2, 2, 1))
+    B.blocks[(0, 1, 0)] = rng.uniform(size=(3, 3, 1))
+    B.blocks[(1, 1, 1)] = rng.uniform(size=(2, 3, 3))
+    C = td(A, B, [1], [1])
+    assert sorted(C.blocks) == [(0, 1, 0), (1, 0, 0), (1, 1, 1)]
+    assert np.allclose(C.blocks[(0, 1, 0)], np.tensordot(A.blocks[(0, 0)], B.blocks[(1, 0, 0)], ([1], [1])))
+    assert np.allclose(C.blocks[(1, 0, 0)], np.tensordot(A.blocks[(1, 1)], B.blocks[(0, 1, 0)], ([1], [1])))
+    assert np.allclose(C.blocks[(1, 1, 1)], np.tensordot(A.blocks[(1, 1)], B.blocks[(1, 1, 1)], ([1], [1])))
+    tr = td(A, A.inverse_cvals(), [0, 1], [0, 1])
+    assert np.isclose(float(tr.blocks[()]), sum(float(np.sum(b * b)) for b in A.blocks.values()))
+
+
+def _trunc_case():
+    """U (4 x 4 sections, one group with every charge 0) holding the blocks
+    (0,3), (1,2), (3,3); d with one value per sector, sectors 2 and 3 below
+    the cut."""
+    z = [(0,)] * 4
+    U = BTensor((0,), [[1] * 4, [1] * 4], [z, z], (0,))
+    for k in ((0, 3), (1, 2), (3, 3)):
+        U.blocks[k] = np.ones((1, 1))
+    d = BTensor((0,), [[1] * 4], [z], (0,))
+    for i, v in enumerate((1.0, 0.9, 1e-9, 1e-9)):
+        d.blocks[(i,)] = np.array([v])
+    return U, d
+
+
+def test_truncate_erasures_follow_the_unsorted_list_known_answer():
+    """remove_unit_blocks (sources/btensor_linalg.cpp:690-709) traced by hand.
+    Sectors are visited last to first.  Erasing sector 3 from
+    [(0,3), (1,2), (3,3)]: the first block matches and is swapped with the
+    second, the third matches at the end, so the list becomes
+    [(1,2), (0,3)] (a match directly after a match survives, and the list is
+    no longer sorted).  Erasing sector 2 from that list: (1,2) matches and is
+    swapped past (0,3); the result is [(0,3)].  Walking the re-sorted list
+    [(0,3), (1,2)] instead would keep both."""
+    U, d = _trunc_case()
+    d2, (U2,), _ = truncate_impl(d, [U], 10 ** 9, 1, 1e-8)
+    assert sorted(d2.blocks) == [(0,), (1,)]
+    assert list(U2.blocks) == [(0, 3)]
+

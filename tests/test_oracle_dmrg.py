@@ -155,3 +155,100 @@ def test_dmrg_matches_exact(name):
         # free fermions with hopping 1/2 (S+S- + h.c.)/2: eps_k = cos(pi k/(L+1)), half filling
         eps = np.cos(np.pi * np.arange(1, L + 1) / (L + 1))
         assert abs(e0 - np.sort(eps)[:L // 2].sum()) < 1e-12
+
+
+@pytest.mark.parametrize("oc", [9, 4])
+def test_dmrg_from_other_centre_matches_exact(oc):
+    """dmrg_impl from a centre other than 0 (sources/dmrg.cpp:224-236,
+    :257-266): the sweeps converge to the exact ground state, and the centre
+    ends where it started (one to the left for the last site)."""
+    L, dense = 10, dor.heisenberg_dense(-1.0, 10)
+    bmpo = dor.guess_bmpo(dor.drop_zero_channels(dense), dor.u1_spin_half_phys())
+    mps, _ = dor.move_oc(dor.prepare_state(dor.random_bmps(4, bmpo, (0,), 1)), 0, oc)
+    E, out, Es = dor.dmrg(bmpo, mps, dor.DmrgOptions(cutoff=1e-12, convergence_criterion=1e-13, maximum_bond=64,
+                                                      maximum_iterations=12), oc=oc)
+    e0 = _ed_sz0(dense, L)
+    assert abs(E - e0) < 1e-9 * abs(e0)
+    # the sweeps end on the start position; after a final backward update the
+    # singular values sit one site to the right of it (the reference's own
+    # comment at sources/dmrg.cpp:259-260)
+    end = oc - 1 if oc == L - 1 else oc
+    centre = end if oc in (0, L - 1) else end + 1
+    for i in range(centre + 1, L):   # right-canonical beyond the centre
+        D = out[i].to_dense().reshape(out[i].to_dense().shape[0], -1)
+        assert np.abs(D @ D.T - np.eye(D.shape[0])).max() < 1e-10
+    for i in range(centre):          # left-canonical before it
+        D = out[i].to_dense()
+        M = D.reshape(-1, D.shape[2])
+        assert np.abs(M.T @ M - np.eye(M.shape[1])).max() < 1e-10
+
+
+def _random_update(L=8, seed=3, bond=6):
+    """theta, W12, environments at the left edge of a random-field XXZ chain
+    (a non-degenerate two-site problem)."""
+    rng = np.random.default_rng(seed)
+    mpo = dor.guess_bmpo(dor.drop_zero_channels(dor.xxz_field_dense(0.7, rng.uniform(-1, 1, L))),
+                         dor.u1_spin_half_phys())
+    mps = dor.prepare_state(dor.random_bmps(bond, mpo, (0,), seed))
+    env = dor.generate_env(mpo, mps, 0)
+    W12 = dor.two_site_hamil(mpo)
+    theta = tensordot(mps[0], mps[1], [2], [0])
+    return theta, W12[0], env[-1], env[2]
+
+
+def test_lanczos_two_vectors_is_the_one_step_update():
+    """k-step Lanczos with two Krylov vectors: Ritz value = eig2x2's E of the
+    one-step form (sources/dmrg.cpp:543-608), Ritz vector = o theta + n psi1."""
+    from oracle.btensor_oracle import add as badd, scal as bscal
+    theta, W, Le, Re = _random_update()
+    E1, new1, _ = dor.two_sites_update(theta, W, Le, Re)
+    E2, new2, nv = dor.lanczos_ground(theta, W, Le, Re, 2, 0.0)
+    assert nv == 2
+    assert abs(E1 - E2) <= 1e-12 * abs(E1)
+    assert sorted(new1.blocks) == sorted(new2.blocks)
+    diff = badd(new1, new2, -1.0)
+    assert np.sqrt(sum(float(np.sum(b * b)) for b in diff.blocks.values())) <= 1e-10
+
+
+def test_lanczos_reaches_dense_two_site_ground_state():
+    """The dense effective Hamiltonian of the two-site problem, built by
+    applying the matvec to every unit vector of theta's blocks: its lowest
+    eigenvalue equals the converged Ritz value, and the Ritz vector is its
+    eigenvector (residual below tolerance)."""
+    theta, W, Le, Re = _random_update()
+    E, vec, nv = dor.lanczos_ground(theta, W, Le, Re, 40, 1e-12)
+    keys = sorted(theta.blocks)
+    sizes = [theta.blocks[k].size for k in keys]
+    n = sum(sizes)
+    H = np.zeros((n, n))
+    col = 0
+    for k, sz in zip(keys, sizes):
+        for e in range(sz):
+            u = theta.copy()
+            u.blocks = {kk: np.zeros_like(v) for kk, v in theta.blocks.items()}
+            u.blocks[k].reshape(-1)[e] = 1.0
+            hu = dor.matvec(u, W, Le, Re)
+            H[:, col] = np.concatenate([hu.blocks[kk].reshape(-1) if kk in hu.blocks else np.zeros(s)
+                                        for kk, s in zip(keys, sizes)])
+            col += 1
+    assert np.abs(H - H.T).max() <= 1e-12 * np.abs(H).max()
+    e0 = np.linalg.eigvalsh(H)[0]
+    assert abs(E - e0) <= 1e-12 * abs(e0)
+    assert set(vec.blocks) == set(keys) or set(vec.blocks) >= set(keys)
+    x = np.concatenate([vec.blocks[k].reshape(-1) for k in keys])
+    assert np.linalg.norm(H @ x - e0 * x) <= 1e-10 * abs(e0)
+
+
+@pytest.mark.parametrize("name", ["heisenberg10", "xx12"])
+def test_dmrg_with_lanczos_matches_exact(name):
+    if name == "heisenberg10":
+        L, dense = 10, dor.heisenberg_dense(-1.0, 10)
+    else:
+        L, dense = 12, dor.xxz_dense(0.0, 12)
+    bmpo = dor.guess_bmpo(dor.drop_zero_channels(dense), dor.u1_spin_half_phys())
+    mps = dor.prepare_state(dor.random_bmps(4, bmpo, (0,), 0))
+    E, _, Es = dor.dmrg(bmpo, mps, dor.DmrgOptions(cutoff=1e-12, convergence_criterion=1e-12, maximum_bond=64,
+                                                    maximum_iterations=8, max_krylov=30, lanczos_tol=1e-12))
+    e0 = _ed_sz0(dense, L)
+    assert abs(E - e0) < 1e-10 * abs(e0)
+    assert len(Es) <= 5   # converged Lanczos: the energy settles within a few sweeps

@@ -65,3 +65,17 @@ def test_solver_failure_is_reported_per_request():
     srv = serving.Server(n_slots=2, hbm_budget=1 << 40, solve_fn=failing, context_fn=lambda n: object())
     res, _ = srv.run(trace)
     assert isinstance(res[1], ArithmeticError) and res[0].energy == 0.5 and res[2].energy == 0.5
+
+
+def test_unconverged_solves_are_failures():
+    """A solve that stops at the sweep cap short of the criterion is reported
+    as unconverged and left out of solves/s and the latency percentiles."""
+    def capped(ctx, req):
+        return -1.0, 40, req.rid % 2 == 0
+
+    trace = [serving.Request(i, 0.0, 16, 64, 1.0, i) for i in range(4)]
+    srv = serving.Server(n_slots=2, hbm_budget=1 << 40, solve_fn=capped, context_fn=lambda n: object())
+    res, wall = srv.run(trace)
+    s = serving.summarize(res, wall)
+    assert s["solves"] == 2 and s["unconverged"] == 2 and s["failed"] == 0
+    assert [r.converged for r in res] == [True, False, True, False]
