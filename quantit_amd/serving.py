@@ -51,6 +51,7 @@ class Result:
     finish: float
     arrival: float
     slot: int
+    converged: bool = True   # |dE/E| between the last two sweeps within the criterion
 
     @property
     def latency(self):
@@ -85,7 +86,9 @@ def make_trace(n, rate, seed=5, L_range=(16, 128), D_choices=(64, 128, 256, 512)
 
 def solve(ctx, req, cutoff=1e-10, criterion=1e-10, max_sweeps=40):
     """One ground-state solve on the given context: XXZ chain with open ends,
-    U(1) Sz = 0, random bMPS start, two-site DMRG to |dE/E| <= criterion."""
+    U(1) Sz = 0, random bMPS start, two-site DMRG to |dE/E| <= criterion
+    between sweeps (the reference's test, sources/dmrg.cpp:247-254).
+    Returns (energy, sweeps, converged)."""
     dense = models.spin_mpo(req.L, models.chain_bonds(req.L), J=1.0, delta=req.delta)
     mpo = models.to_bmpo(ctx, dense)
     mps = mps_prepare(models.random_bmps(ctx, req.L, 4, q=0, seed=req.seed))
@@ -93,14 +96,21 @@ def solve(ctx, req, cutoff=1e-10, criterion=1e-10, max_sweeps=40):
                        maximum_iterations=max_sweeps)
     E, mps, log = dmrg(mpo, mps, opts)
     ctx.sync()
-    return E, len(log["energies"])
+    es = log["energies"]
+    converged = len(es) >= 2 and abs((es[-1] - es[-2]) / es[-1]) <= criterion
+    return E, len(es), converged
 
 
 class Server:
     """n_slots solve slots sharing an HBM budget (bytes)."""
 
-    def __init__(self, n_slots=4, hbm_budget=200 << 30, device=0, solve_fn=None, context_fn=None, **solve_kw):
+    def __init__(self, n_slots=4, hbm_budget=200 << 30, device=0, solve_fn=None, context_fn=None, max_krylov=0,
+                 **solve_kw):
+        """max_krylov: 0 runs the reference's one-step Lanczos update; 2..64
+        the k-step Lanczos eigensolver (a solve then converges in a few
+        sweeps)."""
         self.n_slots = n_slots
+        self.max_krylov = max_krylov
         self.budget = hbm_budget
         self.device = device
         self.solve_kw = solve_kw
@@ -115,6 +125,8 @@ class Server:
         # several solves at once: one stream each (a second stream per solve
         # would share hardware queues with the other solves' streams)
         ctx.set_overlap(self.n_slots == 1)
+        if self.max_krylov:
+            ctx.set_lanczos(self.max_krylov, 1e-12)
         return ctx
 
     def run(self, trace, time_scale=1.0, progress=None):
@@ -163,11 +175,15 @@ class Server:
                 ctx = None
                 try:
                     ctx = self.context_fn(need)
-                    E, sweeps = self.solve_fn(ctx, r, **self.solve_kw)
+                    got = self.solve_fn(ctx, r, **self.solve_kw)
+                    E, sweeps = got[0], got[1]
+                    conv = bool(got[2]) if len(got) > 2 else True
                     results[r.rid] = Result(r.rid, E, sweeps, start, time.perf_counter() - t0,
-                                            r.arrival * time_scale, slot)
+                                            r.arrival * time_scale, slot, conv)
                 except Exception as e:  # reported per request, the server goes on
-                    results[r.rid] = e
+                    # without its traceback, which would keep the solve's
+                    # frames (and their device tensors) alive
+                    results[r.rid] = e.with_traceback(None)
                 finally:
                     if ctx is not None and hasattr(ctx, "close"):
                         ctx.close()
@@ -187,9 +203,13 @@ class Server:
 
 
 def summarize(results, wall):
-    ok = [r for r in results if isinstance(r, Result)]
+    """Counts, throughput and latency percentiles of a served trace.  A solve
+    that ended at its sweep cap without meeting the convergence criterion is
+    a failure (``unconverged``), not a served result."""
+    done = [r for r in results if isinstance(r, Result)]
+    ok = [r for r in done if r.converged]
     lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
-    return {"solves": len(ok), "failed": len(results) - len(ok), "wall_s": wall,
-            "solves_per_s": len(ok) / wall if wall > 0 else 0.0,
+    return {"solves": len(ok), "failed": len(results) - len(done), "unconverged": len(done) - len(ok),
+            "wall_s": wall, "solves_per_s": len(ok) / wall if wall > 0 else 0.0,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "mean_sweeps": float(np.mean([r.sweeps for r in ok])) if ok else 0.0}
+            "mean_sweeps": float(np.mean([r.sweeps for r in done])) if done else 0.0}

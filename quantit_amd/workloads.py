@@ -73,3 +73,29 @@ def contract_svd(D=1024):
     X = random_parts([dl, dl, PHYS, wl], (0,), seed=1)
     A = random_parts([inverse(dl), inverse(PHYS), dl], (0,), seed=2)
     return X, A, [1, 2], [0, 1], 1
+
+
+def _loguniform_ints(rng, n, lo=4, hi=768):
+    return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), n))).astype(int).clip(lo, hi)
+
+
+def grouped_batch(seed=2, n_gemm=2048, n_svd=512):
+    """Grouped batch of independent blocks as block-diagonal btensors: item i
+    gets its own U(1) charge i, so A(m_i x k_i) . B(k_i x n_i) are the only
+    matching block pairs of one tensordot, and S(m_i x n_i) are the sectors
+    of one block SVD.  Dimensions log-uniform integers in [4, 768]."""
+    rng = np.random.default_rng(seed)
+    m, n, k = (_loguniform_ints(rng, n_gemm) for _ in range(3))
+    sm, sn = (_loguniform_ints(rng, n_svd) for _ in range(2))
+
+    def diag(rows, cols, fill):
+        return dict(groups=(0,), secs=[list(map(int, rows)), list(map(int, cols))],
+                    cvals=[[(i,) for i in range(len(rows))], [(-i,) for i in range(len(cols))]], sel=(0,),
+                    blocks={(i, i): fill(int(r), int(c)) for i, (r, c) in enumerate(zip(rows, cols))})
+
+    A = diag(m, k, lambda r, c: rng.standard_normal((r, c)))
+    B = dict(groups=(0,), secs=[list(map(int, k)), list(map(int, n))],
+             cvals=[[(i,) for i in range(n_gemm)], [(-i,) for i in range(n_gemm)]], sel=(0,),
+             blocks={(i, i): rng.standard_normal((int(k[i]), int(n[i]))) for i in range(n_gemm)})
+    S = diag(sm, sn, lambda r, c: rng.standard_normal((r, c)))
+    return A, B, S, (m, n, k), (sm, sn)
