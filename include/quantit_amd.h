@@ -117,6 +117,21 @@ int qtt_tensordot(const qtt_btensor* a, const qtt_btensor* b, int ndims, const i
                   const int64_t* dims_b, qtt_btensor** out);
 /* Host-counted algorithmic FLOPs (sum of 2 M N K) of the last tensordot on this thread. */
 double qtt_last_tensordot_flops(void);
+/* No counterpart in the reference: qtt_tensordot for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The result is written into `out`, an
+ * existing tensor of the same context with exactly the result's structure
+ * (sections, charges, selection rule, block list in order), for instance a
+ * previous result of the same contraction; it must not share storage with a
+ * or b.  Packed operands, split-K partials and launch descriptors live in a
+ * static device scratch of 128 MiB per GPU; a contraction that needs more
+ * fails with QTT_INVALID_ARGUMENT before any launch.  The call allocates
+ * nothing, records no profiling events and always uses the grouped kernels.
+ * Every call on one GPU shares that scratch: calls that overlap on one GPU
+ * (several contexts driven from several threads) race. */
+int qtt_tensordot_into(const qtt_btensor* a, const qtt_btensor* b, int ndims, const int64_t* dims_a,
+                       const int64_t* dims_b, qtt_btensor* out);
+
 /* ---- elementwise ------------------------------------------------------------ */
 /* broadcast mul (btensor.h:547, sources/btensor.cpp:1750) */
 int qtt_mul(const qtt_btensor* x, const qtt_btensor* y, qtt_btensor** out);

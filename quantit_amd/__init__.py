@@ -99,6 +99,7 @@ SIGNATURES = {
     "qtt_conj": (C.c_int, [P, PP]),
     "qtt_tensordot": (C.c_int, [P, P, C.c_int, I64P, I64P, PP]),
     "qtt_last_tensordot_flops": (C.c_double, []),
+    "qtt_tensordot_into": (C.c_int, [P, P, C.c_int, I64P, I64P, P]),
     "qtt_mul": (C.c_int, [P, P, PP]),
     "qtt_add": (C.c_int, [P, P, C.c_double, PP]),
     "qtt_full_dot": (C.c_int, [P, P, DP]),
@@ -328,6 +329,16 @@ class BTensor:
         db = np.array(list(dims_other) or [0], dtype=np.int64)
         return self._new(library().qtt_tensordot, self.h, other.h, len(list(dims_self)), _arr(da, C.c_int64),
                          _arr(db, C.c_int64))
+
+    def tensordot_into(self, other, dims_self, dims_other, out):
+        """The contraction written into ``out`` (an existing tensor with the
+        result's structure) with no device allocation: temporaries live in
+        the library's static per-GPU scratch.  Returns ``out``."""
+        da = np.array(list(dims_self) or [0], dtype=np.int64)
+        db = np.array(list(dims_other) or [0], dtype=np.int64)
+        _check(library().qtt_tensordot_into(self.h, other.h, len(list(dims_self)), _arr(da, C.c_int64),
+                                            _arr(db, C.c_int64), out.h))
+        return out
 
     def permute(self, perm):
         p = np.array(list(perm), dtype=np.int64)

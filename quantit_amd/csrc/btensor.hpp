@@ -84,6 +84,11 @@ struct ContractStats {
 };
 BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da,
                   const std::vector<int64_t>& db, ContractStats* stats = nullptr);
+// tensordot written into an existing tensor that has the result's structure
+// (same sections, charges, selection rule and block list); every temporary
+// comes from the device's static scratch, so nothing is allocated.
+void tensordot_into(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da,
+                    const std::vector<int64_t>& db, BTensor& out, ContractStats* stats = nullptr);
 bool check_product_compat(const Shape& a, const Shape& b, const std::vector<int64_t>& da,
                           const std::vector<int64_t>& db, bool throws);
 

@@ -373,6 +373,26 @@ int qtt_tensordot(const qtt_btensor* a, const qtt_btensor* b, int ndims, const i
 
 double qtt_last_tensordot_flops(void) { return g_last_flops; }
 
+int qtt_tensordot_into(const qtt_btensor* a, const qtt_btensor* b, int ndims, const int64_t* dims_a,
+                       const int64_t* dims_b, qtt_btensor* out) {
+  return guard([&] {
+    const qtt::BTensor& ta = live(a, "a");
+    const qtt::BTensor& tb = live(b, "b");
+    live(out, "out");
+    same_context(ta, tb);
+    same_context(ta, out->t);
+    if (ndims < 0) throw qtt::InvalidArgument("negative number of contracted dimensions");
+    if (ndims) {
+      need(dims_a, "dims_a");
+      need(dims_b, "dims_b");
+    }
+    std::vector<int64_t> da(dims_a, dims_a + ndims), db(dims_b, dims_b + ndims);
+    qtt::ContractStats st;
+    qtt::tensordot_into(ta, tb, da, db, out->t, &st);
+    g_last_flops = st.flops;
+  });
+}
+
 int qtt_mul(const qtt_btensor* x, const qtt_btensor* y, qtt_btensor** out) {
   return guard([&] {
     const qtt::BTensor& tx = live(x, "x");

@@ -36,6 +36,13 @@ constexpr int kGemmTile = 64;
 void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,
               const GemmPair* pairs, double alpha, double beta);
 
+// Size of the file-scope device scratch (kernels/gemm.hip) that the
+// allocation-free contraction takes its packed operands, split-K partials
+// and descriptors from.
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+// Device address of that scratch on `device` (which must be current):
+// hipGetSymbolAddress on the first call per device, cached afterwards.
+hipError_t noalloc_scratch_address(int device, void** out);
 // hi/lo split FP32 MFMA: each operand x = hi + lo (hi = fp32(x), lo =
 // fp32(x - hi)); products hi*hi + hi*lo + lo*hi accumulate in FP32 per K
 // chunk and are folded into FP64 accumulators after every chunk.

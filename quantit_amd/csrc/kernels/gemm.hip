@@ -11,6 +11,9 @@
 // quadrant = 2x2 MFMA tiles; LDS holds a 64x16 A tile and a 16x64 B tile
 // (k-major, row pitch 80 doubles so the two 16-lane halves of a 32-lane
 // LDS read phase fall in opposite bank halves).
+#include <map>
+#include <mutex>
+
 #include "../kernels.hpp"
 
 namespace qtt {
@@ -373,6 +376,30 @@ void gemm_split(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, 
                 const GemmPair* pairs, double alpha, double beta) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_gemm_split, dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
+}
+
+// Scratch of the allocation-free contraction (packed operands, split-K
+// partials, launch descriptors).  Every GPU the library runs on holds its
+// own copy for the life of the process.  In host code this name stands for
+// the variable, not for a device address: host code uses only the address
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch_address(int device, void** out) {
+  static std::mutex mu;
+  static std::map<int, void*> addr;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  addr[device] = p;
+  *out = p;
+  return hipSuccess;
 }
 
 }  // namespace kern

@@ -75,6 +75,8 @@ void Arena::release(size_t off) {
 
 Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(device) {
   QTT_HIP(hipSetDevice(device));
+  // library set-up on this device: the address of its static scratch
+  QTT_HIP(kern::noalloc_scratch_address(device, &noalloc_scratch_));
   QTT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   QTT_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
   QTT_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
@@ -132,6 +134,11 @@ void* Context::upload(const void* data, size_t bytes) {
   return d;
 }
 
+void Context::stage_to(void* dev_dst, const void* data, size_t bytes) {
+  size_t pos = 0;
+  char* h = stage_host(data, bytes, &pos);
+  if (bytes) QTT_HIP(hipMemcpyAsync(dev_dst, h, bytes, hipMemcpyHostToDevice, stream_));
+}
 
 void Context::fork() {
   if (!side_) QTT_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));

@@ -111,6 +111,13 @@ class Context {
     size_t i = 0;
     return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
   }
+  // Copies host bytes through the pinned staging ring to any device address
+  // (stream-ordered; no allocation).
+  void stage_to(void* dev_dst, const void* data, size_t bytes);
+  // This device's copy of the static scratch of the allocation-free
+  // contraction (its address is taken once per device when the first
+  // context on it is created).
+  void* noalloc_scratch() const { return noalloc_scratch_; }
   void sync();
   // Waits for an event recorded on this stream (a blocking wait: the host
   // thread sleeps instead of polling, which matters when several solve
@@ -188,6 +195,7 @@ class Context {
   size_t read_cap_ = 0;  // words per slot
   hipEvent_t read_ev_[2] = {nullptr, nullptr};
   hipEvent_t sync_ev_ = nullptr;
+  void* noalloc_scratch_ = nullptr;
   char* stage_host(const void* data, size_t bytes, size_t* pos);
 };
 

@@ -99,6 +99,35 @@ class ArenaWork : public TdWork {
   std::vector<BufPtr> keep_;  // released at the end of the call: later work is stream-ordered after it
 };
 
+// Everything in the device's static contraction scratch (no allocation):
+// regions are handed out front to back; the call fails before any launch
+// when they do not fit.
+class StaticWork : public TdWork {
+ public:
+  explicit StaticWork(Context* c) : ctx_(c), base_(static_cast<char*>(c->noalloc_scratch())) {}
+  int64_t take(size_t bytes) override {
+    const size_t off = (pos_ + 255) / 256 * 256;
+    if (off + bytes > kern::kNoAllocScratchBytes)
+      throw InvalidArgument("contraction needs more than the " +
+                            std::to_string(kern::kNoAllocScratchBytes >> 20) +
+                            " MiB scratch of the allocation-free entry point (" + std::to_string(off + bytes) +
+                            " bytes)");
+    pos_ = off + bytes;
+    return reinterpret_cast<double*>(base_ + off) - reinterpret_cast<double*>(ctx_->arena().base());
+  }
+  void* put(const void* src, size_t bytes) override {
+    double* p = reinterpret_cast<double*>(ctx_->arena().base()) + take(bytes ? bytes : 1);
+    ctx_->stage_to(p, src, bytes);
+    return p;
+  }
+  bool timed() const override { return false; }
+
+ private:
+  Context* ctx_;
+  char* base_;
+  size_t pos_ = 0;
+};
+
 // Several descriptor lists in one copy; each list starts 16-byte aligned.
 template <class... T>
 std::tuple<T*...> put_pack(TdWork& w, const std::vector<T>&... v) {
@@ -118,7 +147,8 @@ std::tuple<T*...> put_pack(TdWork& w, const std::vector<T>&... v) {
   return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
 }
 
-// A kernel timer that records only when the work allows it.
+// A kernel timer that records only when the work allows it (the
+// allocation-free path records no events).
 struct MaybeTimer {
   std::unique_ptr<KernelTimer> t;
   MaybeTimer(Context& c, const char* name, const TdWork& w) {
@@ -135,6 +165,17 @@ BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>
                   ContractStats* stats) {
   ArenaWork w(a.ctx);
   return tensordot_impl(a, b, da, db, stats, w, nullptr);
+}
+
+void tensordot_into(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da,
+                    const std::vector<int64_t>& db, BTensor& out, ContractStats* stats) {
+  if (out.ctx != a.ctx) throw InvalidArgument("tensors belong to different contexts");
+  if (!out.storage && out.numel()) throw InvalidArgument("output tensor has no storage");
+  // the output must not share memory with an operand
+  if (out.storage && (out.storage == a.storage || out.storage == b.storage))
+    throw InvalidArgument("output tensor shares storage with an operand");
+  StaticWork w(a.ctx);
+  tensordot_impl(a, b, da, db, stats, w, &out);
 }
 
 namespace {
