@@ -492,6 +492,7 @@ def bench_dmrg(args):
     ctx.set_per_block(args.per_block)
     if args.lanczos:
         ctx.set_lanczos(args.lanczos, 1e-12)
+    ctx.set_graphs(not args.no_graphs)
     mpo_parts, mps_parts = dmrg_inputs(w)
     opts = dmrg_opts(qa, w, sweeps)
     solve(qa, ctx, mpo_parts, mps_parts, qa.DmrgOptions(cutoff=1e-10, convergence_criterion=0.0, maximum_bond=32,
@@ -505,7 +506,8 @@ def bench_dmrg(args):
            "config": {"workload": f"{args.workload}: {w['desc']}; value = last sweep", "precision": args.precision,
                       "per_block_launch": args.per_block, "lanczos": args.lanczos},
            "energy": r["E"], "sweep_energies": r["log"]["energies"], "sweep_ms": ms, "max_bond": r["log"]["max_bond"],
-           "solve_wall_s": r["wall"], "roofline": None, "cpu_baseline": None}
+           "solve_wall_s": r["wall"], "contract_gflop": r["log"]["contract_flops"] / 1e9,
+           "svd_model_gflop": r["log"]["svd_flops"] / 1e9, "roofline": None, "cpu_baseline": None}
     if args.lanczos:
         out["lanczos_vectors"] = ctx.lanczos_stats()
     if p:
@@ -565,6 +567,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only: FP64 batched, profiled solve, CPU baseline")
     ap.add_argument("--profile", action="store_true", help="single workloads: add a profiled solve")
     ap.add_argument("--lanczos", type=int, default=0, help="k-step Lanczos with this many Krylov vectors (0: one-step)")
+    ap.add_argument("--no-graphs", action="store_true", help="k-step Lanczos: run every matvec eagerly")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--requests", type=int, default=8, help="serving: trace length per rate")

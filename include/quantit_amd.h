@@ -76,8 +76,14 @@ int qtt_context_svd_flops(qtt_context* ctx, double* all, double* wide);
    the reference; it lets a solve converge to the energy tolerance instead
    of improving by one Lanczos step per update. */
 int qtt_context_set_lanczos(qtt_context* ctx, int max_krylov, double tol);
-/* k-step mode: two-site updates run and Krylov vectors built so far. */
-int qtt_context_lanczos_stats(qtt_context* ctx, int64_t* updates, int64_t* vectors);
+/* k-step mode: the matvec of an update is captured once as a HIP graph and
+   replayed for its later Krylov vectors (1, the default) or run eagerly (0).
+   The results are the same bits either way.  No counterpart in the
+   reference. */
+int qtt_context_set_graphs(qtt_context* ctx, int enable);
+/* k-step mode: two-site updates run, Krylov vectors built and matvecs run as
+   graph replays so far. */
+int qtt_context_lanczos_stats(qtt_context* ctx, int64_t* updates, int64_t* vectors, int64_t* graph_replays);
 
 /* ---- btensor ------------------------------------------------------------
  * Replaces btensor(index_list sections_by_dim, any_quantity_vector c_vals,
@@ -131,6 +137,16 @@ double qtt_last_tensordot_flops(void);
  * (several contexts driven from several threads) race. */
 int qtt_tensordot_into(const qtt_btensor* a, const qtt_btensor* b, int ndims, const int64_t* dims_a,
                        const int64_t* dims_b, qtt_btensor* out);
+
+/* The FP64 contraction GEMM has tile variants (64x64 single-buffered, and
+   64x64, 128x64, 64x128, 128x128 with double-buffered LDS); they give the
+   same bits.  A contraction's shape class (work-weighted mean block size and
+   tile count) picks its variant from a process-wide table that is filled by
+   timing every variant the first time a class is met; the table persists as
+   JSON through these calls.  No counterpart in the reference. */
+int qtt_gemm_tuning_save(const char* path);
+int qtt_gemm_tuning_load(const char* path);
+int qtt_gemm_tuning_size(int64_t* entries);
 
 /* ---- elementwise ------------------------------------------------------------ */
 /* broadcast mul (btensor.h:547, sources/btensor.cpp:1750) */

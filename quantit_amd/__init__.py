@@ -88,7 +88,8 @@ SIGNATURES = {
     "qtt_context_svd_stats": (C.c_int, [P, I64P, I64P, I64P]),
     "qtt_context_svd_flops": (C.c_int, [P, DP, DP]),
     "qtt_context_set_lanczos": (C.c_int, [P, C.c_int, C.c_double]),
-    "qtt_context_lanczos_stats": (C.c_int, [P, I64P, I64P]),
+    "qtt_context_set_graphs": (C.c_int, [P, C.c_int]),
+    "qtt_context_lanczos_stats": (C.c_int, [P, I64P, I64P, I64P]),
     "qtt_btensor_create": (C.c_int, [P, C.c_int, I64P, I64P, C.c_int, U16P, I16P, I16P, C.c_int64, I32P, DP, PP]),
     "qtt_btensor_destroy": (C.c_int, [P]),
     "qtt_btensor_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), I64P, I64P, I64P]),
@@ -100,6 +101,9 @@ SIGNATURES = {
     "qtt_tensordot": (C.c_int, [P, P, C.c_int, I64P, I64P, PP]),
     "qtt_last_tensordot_flops": (C.c_double, []),
     "qtt_tensordot_into": (C.c_int, [P, P, C.c_int, I64P, I64P, P]),
+    "qtt_gemm_tuning_save": (C.c_int, [C.c_char_p]),
+    "qtt_gemm_tuning_load": (C.c_int, [C.c_char_p]),
+    "qtt_gemm_tuning_size": (C.c_int, [I64P]),
     "qtt_mul": (C.c_int, [P, P, PP]),
     "qtt_add": (C.c_int, [P, P, C.c_double, PP]),
     "qtt_full_dot": (C.c_int, [P, P, DP]),
@@ -195,10 +199,15 @@ class Context:
         residual <= tol |E|."""
         _check(library().qtt_context_set_lanczos(self.h, int(max_krylov), float(tol)))
 
+    def set_graphs(self, enable):
+        """k-step mode: replay each update's matvec as a captured HIP graph
+        (default) or run it eagerly."""
+        _check(library().qtt_context_set_graphs(self.h, int(bool(enable))))
+
     def lanczos_stats(self):
-        u, v = C.c_int64(), C.c_int64()
-        _check(library().qtt_context_lanczos_stats(self.h, C.byref(u), C.byref(v)))
-        return {"updates": u.value, "vectors": v.value}
+        u, v, g = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(library().qtt_context_lanczos_stats(self.h, C.byref(u), C.byref(v), C.byref(g)))
+        return {"updates": u.value, "vectors": v.value, "graph_replays": g.value}
 
     def svd_stats(self):
         """Block SVD/eigh health: calls, largest sweep count of one call, and
@@ -361,6 +370,22 @@ class BTensor:
         out = C.c_double()
         _check(library().qtt_full_dot(self.h, other.h, C.byref(out)))
         return out.value
+
+
+def gemm_tuning_save(path):
+    """Writes the process-wide GEMM tile-variant table as JSON."""
+    _check(library().qtt_gemm_tuning_save(str(path).encode()))
+
+
+def gemm_tuning_load(path):
+    """Merges a table written by gemm_tuning_save into the process's."""
+    _check(library().qtt_gemm_tuning_load(str(path).encode()))
+
+
+def gemm_tuning_size():
+    n = C.c_int64()
+    _check(library().qtt_gemm_tuning_size(C.byref(n)))
+    return n.value
 
 
 def last_tensordot_flops():

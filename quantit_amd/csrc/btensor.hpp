@@ -84,6 +84,25 @@ struct ContractStats {
 };
 BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da,
                   const std::vector<int64_t>& db, ContractStats* stats = nullptr);
+// A contraction planned once and launched any number of times (for graph
+// capture): its launch descriptors and temporaries live in arena regions the
+// plan owns; out is the result tensor every launch writes.
+struct TdPlan {
+  struct Launch;
+  BTensor out;
+  std::vector<BufPtr> keep;
+  std::shared_ptr<Launch> launch;
+};
+TdPlan plan_tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da,
+                      const std::vector<int64_t>& db, ContractStats* stats = nullptr);
+// Enqueues the plan's kernels on its context's stream (timed: per-kernel
+// profiling events, which must be off under stream capture).
+void run_tensordot(const TdPlan& p, bool timed);
+// The process-wide table of FP64 GEMM tile variants per contraction shape
+// class (filled by timing the variants on first use), as JSON.
+void gemm_tuning_save(const std::string& path);
+void gemm_tuning_load(const std::string& path);
+size_t gemm_tuning_size();
 // tensordot written into an existing tensor that has the result's structure
 // (same sections, charges, selection rule and block list); every temporary
 // comes from the device's static scratch, so nothing is allocated.

@@ -211,11 +211,19 @@ int qtt_context_set_lanczos(qtt_context* c, int max_krylov, double tol) {
   });
 }
 
-int qtt_context_lanczos_stats(qtt_context* c, int64_t* updates, int64_t* vectors) {
+int qtt_context_set_graphs(qtt_context* c, int enable) {
+  return guard([&] {
+    need(c, "ctx");
+    c->ctx.graphs = enable != 0;
+  });
+}
+
+int qtt_context_lanczos_stats(qtt_context* c, int64_t* updates, int64_t* vectors, int64_t* graph_replays) {
   return guard([&] {
     need(c, "ctx");
     if (updates) *updates = c->ctx.lanczos_updates;
     if (vectors) *vectors = c->ctx.lanczos_vectors;
+    if (graph_replays) *graph_replays = c->ctx.graph_replays;
   });
 }
 
@@ -390,6 +398,27 @@ int qtt_tensordot_into(const qtt_btensor* a, const qtt_btensor* b, int ndims, co
     qtt::ContractStats st;
     qtt::tensordot_into(ta, tb, da, db, out->t, &st);
     g_last_flops = st.flops;
+  });
+}
+
+int qtt_gemm_tuning_save(const char* path) {
+  return guard([&] {
+    need(path, "path");
+    qtt::gemm_tuning_save(path);
+  });
+}
+
+int qtt_gemm_tuning_load(const char* path) {
+  return guard([&] {
+    need(path, "path");
+    qtt::gemm_tuning_load(path);
+  });
+}
+
+int qtt_gemm_tuning_size(int64_t* entries) {
+  return guard([&] {
+    need(entries, "entries");
+    *entries = int64_t(qtt::gemm_tuning_size());
   });
 }
 

@@ -209,6 +209,7 @@ BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
     }
     ctx->lanczos_updates += 1;
     ctx->lanczos_vectors += lr.iterations;
+    ctx->graph_replays += lr.graph_replays;
     return lr.E;
   }
   // one_step_lanczos (dmrg.cpp:584-608), eig2x2Mat (dmrg.cpp:543-572) on device

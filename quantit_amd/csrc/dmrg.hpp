@@ -84,6 +84,7 @@ struct LanczosResult {
   BTensor E;      // rank-0 Ritz value on device
   double energy = 0, residual = 0;
   int iterations = 0;
+  int64_t graph_replays = 0;  // matvecs run as a replayed HIP graph
 };
 LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W12, const BTensor& L, const BTensor& R,
                              int max_krylov, double tol, ContractStats* cs = nullptr);

@@ -245,6 +245,199 @@ __global__ __launch_bounds__(256) void k_gemm_f64(double* __restrict__ base, con
   store_tile<true>(base + o.c_off, o, m0, n0, wm, wn, lane, acc, alpha, beta);
 }
 
+// ---- tile variants of the FP64 grouped GEMM ---------------------------------
+// TM x TN output tile per work-group (4 waves in 2 x 2, each (TM/2) x (TN/2)
+// as (TM/32) x (TN/32) MFMA 16x16 tiles), k in steps of BK = 16 through a
+// double-buffered LDS: the global loads of step t+1 are in registers while
+// the MFMAs of step t run, then written to the other buffer, one barrier per
+// step.  Every output element accumulates over k in the same order as
+// k_gemm_f64 (16-wide steps, 4-wide MFMA chunks, pairs in list order), so all
+// variants give the same bits.
+template <int TM, int TN>
+struct TileRegs {
+  double a[TM / 16], b[TN / 16];
+};
+
+template <int TM, int TN>
+__device__ __forceinline__ void fetch_t(const double* __restrict__ base, const GemmPair& pr, int M, int N, int m0,
+                                        int n0, int k0, int tid, TileRegs<TM, TN>& r) {
+  constexpr int EA = TM / 16, EB = TN / 16;  // elements per thread
+  const double* A = base + pr.a_off;
+  const double* B = base + pr.b_off;
+  const int K = pr.K;
+  if (pr.a_cs == 1) {  // k contiguous: 16/EA threads per row, EA consecutive k each
+    constexpr int TPR = 16 / EA;
+    const int m = tid / TPR, kq = (tid % TPR) * EA, gm = m0 + m;
+#pragma unroll
+    for (int j = 0; j < EA; ++j) {
+      const int gk = k0 + kq + j;
+      r.a[j] = (gm < M && gk < K) ? A[int64_t(gm) * pr.a_rs + gk] : 0.0;
+    }
+  } else {  // m contiguous (or general): TM/EA threads per k, EA consecutive m each
+    constexpr int TPK = TM / EA;
+    const int k = tid / TPK, mq = (tid % TPK) * EA, gk = k0 + k;
+#pragma unroll
+    for (int j = 0; j < EA; ++j) {
+      const int gm = m0 + mq + j;
+      r.a[j] = (gm < M && gk < K) ? A[int64_t(gm) * pr.a_rs + int64_t(gk) * pr.a_cs] : 0.0;
+    }
+  }
+  if (pr.b_cs == 1) {  // n contiguous: TN/EB threads per k, EB consecutive n each
+    constexpr int TPK = TN / EB;
+    const int k = tid / TPK, nq = (tid % TPK) * EB, gk = k0 + k;
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
+      const int gn = n0 + nq + j;
+      r.b[j] = (gn < N && gk < K) ? B[int64_t(gk) * pr.b_rs + gn] : 0.0;
+    }
+  } else {  // k contiguous (or general): 16/EB threads per column
+    constexpr int TPC = 16 / EB;
+    const int n = tid / TPC, kq = (tid % TPC) * EB, gn = n0 + n;
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
+      const int gk = k0 + kq + j;
+      r.b[j] = (gn < N && gk < K) ? B[int64_t(gk) * pr.b_rs + int64_t(gn) * pr.b_cs] : 0.0;
+    }
+  }
+}
+
+template <int TM, int TN>
+__device__ __forceinline__ void stash_t(const GemmPair& pr, int tid, const TileRegs<TM, TN>& r,
+                                        double (*As)[TM + 16], double (*Bs)[TN + 16]) {
+  constexpr int EA = TM / 16, EB = TN / 16;
+  if (pr.a_cs == 1) {
+    constexpr int TPR = 16 / EA;
+    const int m = tid / TPR, kq = (tid % TPR) * EA;
+#pragma unroll
+    for (int j = 0; j < EA; ++j) As[kq + j][m] = r.a[j];
+  } else {
+    constexpr int TPK = TM / EA;
+    const int k = tid / TPK, mq = (tid % TPK) * EA;
+#pragma unroll
+    for (int j = 0; j < EA; ++j) As[k][mq + j] = r.a[j];
+  }
+  if (pr.b_cs == 1) {
+    constexpr int TPK = TN / EB;
+    const int k = tid / TPK, nq = (tid % TPK) * EB;
+#pragma unroll
+    for (int j = 0; j < EB; ++j) Bs[k][nq + j] = r.b[j];
+  } else {
+    constexpr int TPC = 16 / EB;
+    const int n = tid / TPC, kq = (tid % TPC) * EB;
+#pragma unroll
+    for (int j = 0; j < EB; ++j) Bs[kq + j][n] = r.b[j];
+  }
+}
+
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void k_gemm_f64t(double* __restrict__ base, const GemmTile* __restrict__ tiles,
+                                                   const GemmOut* __restrict__ outs,
+                                                   const GemmPair* __restrict__ pairs, double alpha, double beta) {
+  constexpr int WM = TM / 2, WN = TN / 2, FI = WM / 16, FJ = WN / 16;
+  __shared__ double As[2][BK][TM + 16];
+  __shared__ double Bs[2][BK][TN + 16];
+  const GemmTile t = tiles[blockIdx.x];
+  const GemmOut o = outs[t.out];
+  const int m0 = t.tm * TM, n0 = t.tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+  v4d acc[FI][FJ];
+#pragma unroll
+  for (int i = 0; i < FI; ++i)
+#pragma unroll
+    for (int j = 0; j < FJ; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  int p = o.pair_begin;
+  while (p < o.pair_end && pairs[p].K <= 0) ++p;
+  if (p < o.pair_end) {
+    GemmPair pr = pairs[p];
+    int k0 = 0, buf = 0;
+    TileRegs<TM, TN> regs;
+    fetch_t<TM, TN>(base, pr, o.M, o.N, m0, n0, 0, tid, regs);
+    stash_t<TM, TN>(pr, tid, regs, As[0], Bs[0]);
+    __syncthreads();
+    // the step after (p, k0)
+    auto advance = [&](int& q, int& k, GemmPair& g) {
+      k += BK;
+      if (k >= g.K) {
+        k = 0;
+        ++q;
+        while (q < o.pair_end && pairs[q].K <= 0) ++q;
+        if (q < o.pair_end) g = pairs[q];
+      }
+    };
+    int np = p, nk = k0;
+    GemmPair npr = pr;
+    advance(np, nk, npr);
+    bool more = np < o.pair_end;
+    if (more) fetch_t<TM, TN>(base, npr, o.M, o.N, m0, n0, nk, tid, regs);
+    while (true) {
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4), c = lane & 15;
+        double a[FI], b[FJ];
+#pragma unroll
+        for (int i = 0; i < FI; ++i) a[i] = As[buf][kr][wm + i * 16 + c];
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) b[j] = Bs[buf][kr][wn + j * 16 + c];
+#pragma unroll
+        for (int i = 0; i < FI; ++i)
+#pragma unroll
+          for (int j = 0; j < FJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+      if (!more) break;
+      // regs hold step (np, nk): into the other buffer (its last readers
+      // passed the barrier of the previous step), then fetch the step after
+      stash_t<TM, TN>(npr, tid, regs, As[buf ^ 1], Bs[buf ^ 1]);
+      GemmPair cur = npr;
+      int q = np, k = nk;
+      advance(q, k, npr);
+      more = q < o.pair_end;
+      np = q;
+      nk = k;
+      if (more) fetch_t<TM, TN>(base, npr, o.M, o.N, m0, n0, nk, tid, regs);
+      (void)cur;
+      __syncthreads();
+      buf ^= 1;
+      // the step in buffer buf is complete; one more compute round if it
+      // was the last one loaded
+      if (!more) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+          const int kr = kk + (lane >> 4), c = lane & 15;
+          double a[FI], b[FJ];
+#pragma unroll
+          for (int i = 0; i < FI; ++i) a[i] = As[buf][kr][wm + i * 16 + c];
+#pragma unroll
+          for (int j = 0; j < FJ; ++j) b[j] = Bs[buf][kr][wn + j * 16 + c];
+#pragma unroll
+          for (int i = 0; i < FI; ++i)
+#pragma unroll
+            for (int j = 0; j < FJ; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        break;
+      }
+    }
+  }
+  double* C = base + o.c_off;
+#pragma unroll
+  for (int i = 0; i < FI; ++i)
+#pragma unroll
+    for (int j = 0; j < FJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + 4 * r + (lane >> 4);
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        if (row < o.M && col < o.N) {
+          double* pc = C + int64_t(row) * o.ldc + col;
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * (*pc);
+          *pc = v;
+        }
+      }
+}
+
 __global__ __launch_bounds__(256) void k_gemm_split(double* __restrict__ base, const GemmTile* __restrict__ tiles,
                                                     const GemmOut* __restrict__ outs,
                                                     const GemmPair* __restrict__ pairs, double alpha,
@@ -370,6 +563,33 @@ void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, co
               const GemmPair* pairs, double alpha, double beta) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL(k_gemm_f64, dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
+}
+
+void gemm_f64_variant(hipStream_t s, int variant, double* base, const GemmTile* tiles, int ntiles,
+                      const GemmOut* outs, const GemmPair* pairs, double alpha, double beta) {
+  if (ntiles <= 0) return;
+  switch (variant) {
+    case kTile64:
+      hipLaunchKernelGGL(k_gemm_f64, dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
+      break;
+    case kTile64db:
+      hipLaunchKernelGGL((k_gemm_f64t<64, 64>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
+      break;
+    case kTile128x64:
+      hipLaunchKernelGGL((k_gemm_f64t<128, 64>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
+                         beta);
+      break;
+    case kTile64x128:
+      hipLaunchKernelGGL((k_gemm_f64t<64, 128>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
+                         beta);
+      break;
+    case kTile128:
+      hipLaunchKernelGGL((k_gemm_f64t<128, 128>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
+                         beta);
+      break;
+    default:
+      break;
+  }
 }
 
 void gemm_split(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, const GemmOut* outs,

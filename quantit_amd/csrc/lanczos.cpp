@@ -84,6 +84,67 @@ kern::Copy2D flat_copy(int64_t src, int64_t dst, int64_t ne) {
   return c;
 }
 
+// The matvec L.x.W12.R of one update as a HIP graph: three planned
+// contractions captured once and replayed for every Krylov vector whose
+// layout is the one it was built for.  x is copied into the graph's own input
+// tensor before each replay; the result lands in the last plan's output.
+class MatvecGraph {
+ public:
+  MatvecGraph() = default;
+  MatvecGraph(const MatvecGraph&) = delete;
+  MatvecGraph& operator=(const MatvecGraph&) = delete;
+  ~MatvecGraph() { reset(); }
+  void reset() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    p1_ = p2_ = p3_ = TdPlan();
+    x_ = BTensor();
+    n_ = -1;
+  }
+  bool ready(int64_t layout_n) const { return exec_ && layout_n == n_; }
+  // Plans the three contractions for inputs of `layout` (the blocks of x, in
+  // order) and captures their kernels.
+  void build(const BTensor& shape_of_x, const Layout& lay, const BTensor& W, const BTensor& L, const BTensor& R) {
+    reset();
+    Context* ctx = shape_of_x.ctx;
+    x_ = BTensor(ctx, shape_of_x.shape);
+    x_.allocate(lay.idx);
+    ContractStats cs;
+    p1_ = plan_tensordot(L, x_, {0}, {0}, &cs);
+    p2_ = plan_tensordot(p1_.out, W, {0, 2, 3}, {0, 4, 5}, &cs);
+    p3_ = plan_tensordot(p2_.out, R, {1, 4}, {0, 1}, &cs);
+    flops_ = cs.flops;
+    QTT_HIP(hipStreamBeginCapture(ctx->stream(), hipStreamCaptureModeThreadLocal));
+    run_tensordot(p1_, false);
+    run_tensordot(p2_, false);
+    run_tensordot(p3_, false);
+    QTT_HIP(hipStreamEndCapture(ctx->stream(), &graph_));
+    QTT_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    n_ = lay.n;
+  }
+  // x (row `row` of V, pitch n) -> graph input, then the replay.
+  void launch(Context* ctx, int64_t v_off, ContractStats* cs) {
+    double* ab = reinterpret_cast<double*>(ctx->arena().base());
+    KernelTimer kt(*ctx, "lanczos_matvec_graph");
+    kern::krylov_axpy(ctx->stream(), ab, 0, 0, 0, 0, 0.0, v_off, x_.base() - ab, n_, -1);
+    QTT_HIP(hipGraphLaunch(exec_, ctx->stream()));
+    if (cs) cs->flops += flops_;
+    ++replays;
+  }
+  const BTensor& out() const { return p3_.out; }
+  int64_t replays = 0;
+
+ private:
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  TdPlan p1_, p2_, p3_;
+  BTensor x_;
+  int64_t n_ = -1;
+  double flops_ = 0;
+};
+
 }  // namespace
 
 LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W, const BTensor& Lenv, const BTensor& Renv,
@@ -137,13 +198,26 @@ LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W, const BTens
   LanczosResult res;
   int m = 0;
   double host_out[3] = {0, 0, 0};
+  MatvecGraph graph;
   for (int j = 0; j < K; ++j) {
-    // H v_j through a view of row j
-    BTensor x(ctx, theta.shape);
-    x.storage = V;
-    for (size_t b = 0; b < lay.idx.size(); ++b)
-      x.blocks.push_back({lay.idx[b], int64_t(j) * lay.n + lay.off[b], lay.numel[b]});
-    BTensor o = matvec_labelled(x, W, Lenv, Renv, cs);
+    // H v_j: eagerly through a view of row j, or (from the second vector on,
+    // when graphs are enabled) by replaying the captured matvec of the
+    // current layout
+    BTensor o;
+    if (ctx->graphs && j >= 1) {
+      if (!graph.ready(lay.n)) {
+        BTensor x(ctx, theta.shape);
+        graph.build(x, lay, W, Lenv, Renv);
+      }
+      graph.launch(ctx, voff(j), cs);
+      o = graph.out();
+    } else {
+      BTensor x(ctx, theta.shape);
+      x.storage = V;
+      for (size_t b = 0; b < lay.idx.size(); ++b)
+        x.blocks.push_back({lay.idx[b], int64_t(j) * lay.n + lay.off[b], lay.numel[b]});
+      o = matvec_labelled(x, W, Lenv, Renv, cs);
+    }
     // grow the layout by the output's new blocks
     std::vector<int> where(o.blocks.size());
     bool grow = false;
@@ -229,6 +303,7 @@ LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W, const BTens
     kern::krylov_axpy(ctx->stream(), ab, voff(0), lay.n, 0, nrm2, 0.0, woff(), voff(j + 1), lay.n, nrm2);
   }
   res.iterations = m;
+  res.graph_replays = graph.replays;
   res.energy = host_out[0];
   res.residual = host_out[1];
   // theta' = sum_i y_i v_i, restricted to the blocks the Krylov vectors have

@@ -81,7 +81,8 @@ Context::Context(int device, size_t arena_bytes, size_t stage_bytes) : device_(d
   QTT_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
   QTT_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
   arena_ = std::make_shared<Arena>(arena_bytes);
-  stage_cap_ = round_up(stage_bytes);
+  // the descriptor staging ring comes out of the arena: at most a quarter of it
+  stage_cap_ = round_up(std::min(stage_bytes, std::max<size_t>(size_t(1) << 20, arena_bytes / 4)));
   QTT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_stage_), stage_cap_, hipHostMallocDefault));
   dev_stage_ = alloc(stage_cap_);
   for (auto& e : read_ev_) QTT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));

@@ -156,6 +156,10 @@ class Context {
   int lanczos_krylov = 0;
   double lanczos_tol = 1e-12;
   int64_t lanczos_updates = 0, lanczos_vectors = 0;  // k-step mode: updates and Krylov vectors built
+  // k-step mode: the matvec of an update is captured in a HIP graph and
+  // replayed for the later Krylov vectors (count of replays below)
+  bool graphs = true;
+  int64_t graph_replays = 0;
   // Reference-shaped baseline: one GEMM launch per block pair (mm, addmm_).
   bool per_block_launch = false;
   // Independent work of one call may run on the side stream.  Processes that

@@ -3,6 +3,11 @@
 // (sources/btensor.cpp:1971-2119), and one grouped-GEMM launch on device.
 #include <algorithm>
 #include <climits>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -69,6 +74,67 @@ struct KeyEnt {
   BIdx key;
   int src;
 };
+
+// ---- GEMM tile-variant tuning table (process-wide) -----------------------------
+// A contraction's shape class: log2 of the work-weighted mean M and N of its
+// MFMA outputs and of its 64 x 64 tile count.  Classes with fewer than
+// kTuneMinTiles tiles always take the 64 x 64 tile (latency-bound).
+constexpr int64_t kTuneMinTiles = 64;
+
+struct TuneTable {
+  std::mutex mu;
+  std::map<std::string, int> best;
+};
+TuneTable& tune_table() {
+  static TuneTable t;
+  return t;
+}
+
+int ilog2(double x) { return x < 1.0 ? 0 : int(std::floor(std::log2(x))); }
+
+std::string gemm_shape_key(const std::vector<kern::GemmOut>& outs, const std::vector<int64_t>& work, int64_t tiles) {
+  double wm = 0, wn = 0, w = 0;
+  for (size_t o = 0; o < outs.size(); ++o) {
+    const double f = double(outs[o].M) * outs[o].N * double(std::max<int64_t>(work[o], 1));
+    wm += f * outs[o].M;
+    wn += f * outs[o].N;
+    w += f;
+  }
+  if (w > 0) {
+    wm /= w;
+    wn /= w;
+  }
+  return "f64 m" + std::to_string(ilog2(wm)) + " n" + std::to_string(ilog2(wn)) + " t" +
+         std::to_string(ilog2(double(tiles)));
+}
+
+// Variant for a class.  QTT_GEMM_VARIANT=<n> forces one; QTT_GEMM_TUNE=0
+// disables tuning (64 x 64 unless the table has an entry).  *tune is set
+// when the caller should time the variants now (unknown class, tuning on,
+// timing allowed).
+int gemm_variant_for(const std::string& key, int64_t tiles, bool may_time, bool* tune) {
+  *tune = false;
+  if (const char* f = std::getenv("QTT_GEMM_VARIANT")) {
+    const int v = std::atoi(f);
+    if (v >= 0 && v < kern::kGemmVariants) return v;
+  }
+  if (tiles < kTuneMinTiles) return kern::kTile64;
+  {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> lk(t.mu);
+    auto it = t.best.find(key);
+    if (it != t.best.end()) return it->second;
+  }
+  const char* e = std::getenv("QTT_GEMM_TUNE");
+  *tune = may_time && !(e && std::atoi(e) == 0);
+  return kern::kTile64;
+}
+
+void gemm_variant_record(const std::string& key, int v) {
+  TuneTable& t = tune_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  t.best[key] = v;
+}
 
 // Where one contraction keeps its temporaries (packed operands, split-K
 // partials) and its launch descriptors.
@@ -147,24 +213,156 @@ std::tuple<T*...> put_pack(TdWork& w, const std::vector<T>&... v) {
   return std::tuple<T*...>{reinterpret_cast<T*>(d + offs[i++])...};
 }
 
-// A kernel timer that records only when the work allows it (the
-// allocation-free path records no events).
-struct MaybeTimer {
-  std::unique_ptr<KernelTimer> t;
-  MaybeTimer(Context& c, const char* name, const TdWork& w) {
-    if (w.timed()) t = std::make_unique<KernelTimer>(c, name);
+BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
+                       const std::vector<int64_t>& db_in, ContractStats* stats, TdWork& work, const BTensor* into,
+                       TdPlan::Launch* plan);
+
+// Temporaries and descriptors in arena regions owned by a plan, so the plan
+// can be launched again later (descriptors are copied to the device once).
+class PersistWork : public TdWork {
+ public:
+  PersistWork(Context* c, std::vector<BufPtr>* keep) : ctx_(c), keep_(keep) {}
+  int64_t take(size_t bytes) override {
+    keep_->push_back(ctx_->alloc(bytes));
+    return keep_->back()->dptr() - reinterpret_cast<double*>(ctx_->arena().base());
   }
+  void* put(const void* src, size_t bytes) override {
+    keep_->push_back(ctx_->alloc(bytes ? bytes : 1));
+    ctx_->stage_to(keep_->back()->cptr(), src, bytes);
+    return keep_->back()->cptr();
+  }
+  bool timed() const override { return false; }
+
+ private:
+  Context* ctx_;
+  std::vector<BufPtr>* keep_;
 };
 
-BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
-                       const std::vector<int64_t>& db_in, ContractStats* stats, TdWork& work, const BTensor* into);
+}  // namespace
+
+// Kernel arguments of one contraction: permute-packs, narrow-row tasks, MFMA
+// tiles, split-K sums (device descriptor pointers).
+struct TdPlan::Launch {
+  const kern::PermDesc* pdesc = nullptr;
+  const kern::ChunkTask* ptasks = nullptr;
+  int nptasks = 0;
+  const kern::RowTask* rows = nullptr;
+  int nrows = 0;
+  const kern::GemmTile* tiles = nullptr;
+  int ntiles = 0;
+  const kern::GemmOut* outs = nullptr;
+  const kern::GemmPair* pairs = nullptr;
+  const kern::SumParts* sums = nullptr;
+  const kern::ChunkTask* sum_tasks = nullptr;
+  int nsum_tasks = 0;
+  bool split_fp32 = false;
+  int variant = 0;  // FP64 tile variant (kern::GemmVariant)
+  const char* gemm_name = "gemm_f64";
+};
+
+namespace {
+
+void td_pack(Context* ctx, const TdPlan::Launch& l, bool timed) {
+  if (!l.nptasks) return;
+  std::unique_ptr<KernelTimer> kt;
+  if (timed) kt = std::make_unique<KernelTimer>(*ctx, "pack_permute");
+  kern::permute_copy(ctx->stream(), reinterpret_cast<double*>(ctx->arena().base()), l.pdesc, l.ptasks, l.nptasks);
+}
+
+void td_run(Context* ctx, const TdPlan::Launch& l, bool timed) {
+  double* ab = reinterpret_cast<double*>(ctx->arena().base());
+  auto timer = [&](const char* name) {
+    return timed ? std::make_unique<KernelTimer>(*ctx, name) : std::unique_ptr<KernelTimer>();
+  };
+  td_pack(ctx, l, timed);
+  if (l.nrows) {
+    auto kt = timer("gemm_rows");
+    kern::gemm_rows(ctx->stream(), ab, l.rows, l.nrows, l.outs, l.pairs, 1.0, 0.0);
+  }
+  if (l.split_fp32) {
+    auto kt = timer(l.gemm_name);
+    kern::gemm_split(ctx->stream(), ab, l.tiles, l.ntiles, l.outs, l.pairs, 1.0, 0.0);
+  } else {
+    auto kt = timer(l.gemm_name);
+    kern::gemm_f64_variant(ctx->stream(), l.variant, ab, l.tiles, l.ntiles, l.outs, l.pairs, 1.0, 0.0);
+  }
+  if (l.nsum_tasks) {
+    auto kt = timer("gemm_split_k_sum");
+    kern::sum_parts(ctx->stream(), ab, l.sums, l.sum_tasks, l.nsum_tasks);
+  }
+}
 
 }  // namespace
+
+// The tuning table as JSON: {"gemm_variants": {"<class>": <variant>, ...}}.
+void gemm_tuning_save(const std::string& path) {
+  TuneTable& t = tune_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  std::FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) throw InvalidArgument("cannot write " + path);
+  std::fprintf(f, "{\"gemm_variants\": {");
+  bool first = true;
+  for (auto& kv : t.best) {
+    std::fprintf(f, "%s\n  \"%s\": %d", first ? "" : ",", kv.first.c_str(), kv.second);
+    first = false;
+  }
+  std::fprintf(f, "\n}}\n");
+  std::fclose(f);
+}
+
+void gemm_tuning_load(const std::string& path) {
+  std::FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) throw InvalidArgument("cannot read " + path);
+  std::string text;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+  std::fclose(f);
+  const size_t start = text.find("\"gemm_variants\"");
+  if (start == std::string::npos) throw InvalidArgument(path + ": no gemm_variants table");
+  size_t pos = text.find('{', start);
+  std::map<std::string, int> got;
+  while (pos != std::string::npos) {
+    const size_t q0 = text.find('"', pos + 1);
+    const size_t close = text.find('}', pos + 1);
+    if (q0 == std::string::npos || (close != std::string::npos && close < q0)) break;
+    const size_t q1 = text.find('"', q0 + 1);
+    const size_t colon = text.find(':', q1);
+    if (q1 == std::string::npos || colon == std::string::npos) throw InvalidArgument(path + ": malformed table");
+    const int v = std::atoi(text.c_str() + colon + 1);
+    if (v < 0 || v >= kern::kGemmVariants) throw InvalidArgument(path + ": unknown variant");
+    got[text.substr(q0 + 1, q1 - q0 - 1)] = v;
+    pos = text.find_first_of(",}", colon);
+    if (pos == std::string::npos || text[pos] == '}') break;
+  }
+  TuneTable& t = tune_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  for (auto& kv : got) t.best[kv.first] = kv.second;
+}
+
+size_t gemm_tuning_size() {
+  TuneTable& t = tune_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  return t.best.size();
+}
+
+TdPlan plan_tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da,
+                      const std::vector<int64_t>& db, ContractStats* stats) {
+  TdPlan p;
+  p.launch = std::make_shared<TdPlan::Launch>();
+  PersistWork w(a.ctx, &p.keep);
+  p.out = tensordot_impl(a, b, da, db, stats, w, nullptr, p.launch.get());
+  return p;
+}
+
+void run_tensordot(const TdPlan& p, bool timed) {
+  if (p.launch && p.out.ctx) td_run(p.out.ctx, *p.launch, timed);
+}
 
 BTensor tensordot(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da, const std::vector<int64_t>& db,
                   ContractStats* stats) {
   ArenaWork w(a.ctx);
-  return tensordot_impl(a, b, da, db, stats, w, nullptr);
+  return tensordot_impl(a, b, da, db, stats, w, nullptr, nullptr);
 }
 
 void tensordot_into(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da,
@@ -175,14 +373,16 @@ void tensordot_into(const BTensor& a, const BTensor& b, const std::vector<int64_
   if (out.storage && (out.storage == a.storage || out.storage == b.storage))
     throw InvalidArgument("output tensor shares storage with an operand");
   StaticWork w(a.ctx);
-  tensordot_impl(a, b, da, db, stats, w, &out);
+  tensordot_impl(a, b, da, db, stats, w, &out, nullptr);
 }
 
 namespace {
 
 BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int64_t>& da_in,
-                       const std::vector<int64_t>& db_in, ContractStats* stats, TdWork& work, const BTensor* into) {
+                       const std::vector<int64_t>& db_in, ContractStats* stats, TdWork& work, const BTensor* into,
+                       TdPlan::Launch* plan) {
   Context* ctx = a.ctx;
+  TdPlan::Launch lch;
   HostTimer ht_match(*ctx, work.timed() ? "host_td_match" : nullptr);
   if (a.shape.g != b.shape.g) throw CheckError("the two tensors have different type of conserved quantities");
   if (da_in.size() != db_in.size()) throw CheckError("both dimension lists should have the same length.");
@@ -405,11 +605,9 @@ BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int
       if (v.need_pack) v.off += soff;
     for (auto& v : vb)
       if (v.need_pack) v.off += soff;
-    MaybeTimer kt(*ctx, "pack_permute", work);
     HostTimer ht(*ctx, work.timed() ? "host_td_upload" : nullptr);
-    auto [dd, dt] = put_pack(work, pdesc, ptasks);
-    ht.stop();
-    kern::permute_copy(ctx->stream(), ab, dd, dt, int(ptasks.size()));
+    std::tie(lch.pdesc, lch.ptasks) = put_pack(work, pdesc, ptasks);
+    lch.nptasks = int(ptasks.size());
   }
 
   // grouped GEMM descriptors
@@ -460,7 +658,7 @@ BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int
   // mode); otherwise every block takes MFMA tiles, so a contraction is one
   // launch either way.
   const char* rows_env = std::getenv("QTT_GEMM_ROWS");  // "0": MFMA tiles for every block
-  const bool per_block = ctx->per_block_launch && !into;
+  const bool per_block = ctx->per_block_launch && !into && !plan;
   bool all_narrow = ctx->precision == Precision::FP64 && !per_block && !(rows_env && std::atoi(rows_env) == 0);
   for (size_t o = 0; o < outs.size() && all_narrow; ++o)
     all_narrow = gouts[o].N <= kern::kRowsMaxN && out_work[o] <= kern::kRowsMaxK;
@@ -549,24 +747,40 @@ BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int
   // one's tiles row by row sorts the tiles by work
   std::vector<kern::GemmTile> tiles;
   std::vector<kern::RowTask> row_tasks;
-  tiles.reserve(size_t(mfma_tiles));
-  {
-    std::vector<int> ord(launch_outs.size());
-    std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return launch_work[x] > launch_work[y]; });
+  std::vector<int> ord(launch_outs.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return launch_work[x] > launch_work[y]; });
+  // MFMA tiles of a TM x TN variant, longest work first, row by row
+  auto tiles_of = [&](int variant) {
+    const int TM = kern::kVariantTM[variant], TN = kern::kVariantTN[variant];
+    std::vector<kern::GemmTile> tl;
     for (int o : ord) {
-      if (size_t(o) < outs.size() && narrow[size_t(o)]) {
-        const int rpt = kern::rows_per_task(gouts[o].N);
-        for (int r0 = 0; r0 < gouts[o].M; r0 += rpt) row_tasks.push_back({int32_t(o), int32_t(r0)});
-        continue;
-      }
-      if (size_t(o) < outs.size() && split_block[size_t(o)]) continue;
-      const int tm = (launch_outs[o].M + kern::kGemmTile - 1) / kern::kGemmTile;
-      const int tn = (launch_outs[o].N + kern::kGemmTile - 1) / kern::kGemmTile;
+      if (size_t(o) < outs.size() && (narrow[size_t(o)] || split_block[size_t(o)])) continue;
+      const int tm = (launch_outs[o].M + TM - 1) / TM;
+      const int tn = (launch_outs[o].N + TN - 1) / TN;
       for (int i = 0; i < tm; ++i)
-        for (int j = 0; j < tn; ++j) tiles.push_back({int32_t(o), int16_t(i), int16_t(j)});
+        for (int j = 0; j < tn; ++j) tl.push_back({int32_t(o), int16_t(i), int16_t(j)});
     }
+    return tl;
+  };
+  for (int o : ord)
+    if (size_t(o) < outs.size() && narrow[size_t(o)]) {
+      const int rpt = kern::rows_per_task(gouts[o].N);
+      for (int r0 = 0; r0 < gouts[o].M; r0 += rpt) row_tasks.push_back({int32_t(o), int32_t(r0)});
+    }
+  // tile variant: from the tuning table for this contraction's shape class;
+  // a class not seen yet is timed over every variant when the call may
+  // time (not inside plans or the allocation-free path), else the 64 x 64
+  // tile runs
+  const bool f64_tiles = ctx->precision == Precision::FP64 && !per_block;
+  int variant = kern::kTile64;
+  std::string tune_key;
+  bool tune_now = false;
+  if (f64_tiles) {
+    tune_key = gemm_shape_key(launch_outs, launch_work, mfma_tiles);
+    variant = gemm_variant_for(tune_key, mfma_tiles, work.timed(), &tune_now);
   }
+  tiles = tiles_of(tune_now ? int(kern::kTile64) : variant);
   if (stats) {
     stats->flops += flops;
     stats->bytes += bytes;
@@ -595,6 +809,7 @@ BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int
         launches.push_back({t0, tm * tn});
       }
     }
+    td_pack(ctx, lch, true);
     if (one.empty()) return out;
     auto [d_one, d_tl, d_pairs] = ctx->upload_pack(one, tl, gpairs);
     for (size_t l = 0; l < launches.size(); ++l) {
@@ -607,24 +822,50 @@ BTensor tensordot_impl(const BTensor& a, const BTensor& b, const std::vector<int
   }
   ht_plan.stop();
   HostTimer ht_up(*ctx, work.timed() ? "host_td_upload" : nullptr);
-  auto [d_tiles, d_outs, d_pairs, d_rows, d_sums, d_sum_tasks] =
+  std::tie(lch.tiles, lch.outs, lch.pairs, lch.rows, lch.sums, lch.sum_tasks) =
       put_pack(work, tiles, launch_outs, gpairs, row_tasks, sums, sum_tasks);
   ht_up.stop();
-  if (!row_tasks.empty()) {
-    MaybeTimer kt(*ctx, "gemm_rows", work);
-    kern::gemm_rows(ctx->stream(), ab, d_rows, int(row_tasks.size()), d_outs, d_pairs, 1.0, 0.0);
+  lch.ntiles = int(tiles.size());
+  lch.nrows = int(row_tasks.size());
+  lch.nsum_tasks = int(sum_tasks.size());
+  lch.split_fp32 = ctx->precision == Precision::SPLIT_FP32;
+  lch.variant = variant;
+  lch.gemm_name = ctx->gemm_label ? ctx->gemm_label : (lch.split_fp32 ? "gemm_split" : "gemm_f64");
+  if (tune_now) {
+    // operands ready, then every variant timed on the real launch (each
+    // writes the same output bits); the fastest is recorded and used
+    td_pack(ctx, lch, false);
+    lch.nptasks = 0;
+    float best_ms = 0.f;
+    hipEvent_t e0, e1;
+    QTT_HIP(hipEventCreate(&e0));
+    QTT_HIP(hipEventCreate(&e1));
+    for (int v = 0; v < kern::kGemmVariants; ++v) {
+      std::vector<kern::GemmTile> tl = tiles_of(v);
+      const kern::GemmTile* dt = ctx->upload_vec(tl);
+      float ms = 0.f;
+      for (int rep = 0; rep < 2; ++rep) {
+        QTT_HIP(hipEventRecord(e0, ctx->stream()));
+        kern::gemm_f64_variant(ctx->stream(), v, ab, dt, int(tl.size()), lch.outs, lch.pairs, 1.0, 0.0);
+        QTT_HIP(hipEventRecord(e1, ctx->stream()));
+        QTT_HIP(hipEventSynchronize(e1));
+        QTT_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      if (v == 0 || ms < best_ms) {
+        best_ms = ms;
+        lch.variant = v;
+        lch.tiles = dt;
+        lch.ntiles = int(tl.size());
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    gemm_variant_record(tune_key, lch.variant);
   }
-  if (ctx->precision == Precision::SPLIT_FP32) {
-    MaybeTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_split", work);
-    kern::gemm_split(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
-  } else {
-    MaybeTimer kt(*ctx, ctx->gemm_label ? ctx->gemm_label : "gemm_f64", work);
-    kern::gemm_f64(ctx->stream(), ab, d_tiles, int(tiles.size()), d_outs, d_pairs, 1.0, 0.0);
-  }
-  if (!sums.empty()) {
-    MaybeTimer kt(*ctx, "gemm_split_k_sum", work);
-    kern::sum_parts(ctx->stream(), ab, d_sums, d_sum_tasks, int(sum_tasks.size()));
-  }
+  if (plan)
+    *plan = lch;
+  else
+    td_run(ctx, lch, work.timed());
   return out;
 }
 

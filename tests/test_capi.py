@@ -66,3 +66,21 @@ def test_c_client_compiles_against_the_header(tmp_path):
                           "-L", os.path.join(ROOT, "quantit_amd"), "-lquantit_amd", "-lm"],
                          capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+def test_gemm_tuning_table_json_round_trip(tmp_path):
+    """The tile-variant table loads from and saves to JSON (no GPU needed)."""
+    import quantit_amd as qa
+    src = tmp_path / "in.json"
+    src.write_text('{"gemm_variants": {\n  "f64 m7 n7 t9": 2,\n  "f64 m9 n9 t12": 4\n}}\n')
+    before = qa.gemm_tuning_size()
+    qa.gemm_tuning_load(src)
+    assert qa.gemm_tuning_size() >= before + 0 and qa.gemm_tuning_size() >= 2
+    out = tmp_path / "out.json"
+    qa.gemm_tuning_save(out)
+    text = out.read_text()
+    assert '"f64 m7 n7 t9": 2' in text and '"f64 m9 n9 t12": 4' in text
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"gemm_variants": {"x": 99}}')
+    with pytest.raises(ValueError):
+        qa.gemm_tuning_load(bad)

@@ -191,3 +191,34 @@ def test_lanczos_chain_L32_D128_converges(ctx):
     assert len(es) < 6 or abs((es[-1] - es[-2]) / es[-1]) <= 1e-10
     assert abs((es[-1] - es[-2]) / es[-1]) <= 1e-10
     assert E2 <= E1 + 1e-12 * abs(E1)
+
+
+def test_lanczos_graph_replays_equal_eager(ctx):
+    """k-step mode with each update's matvec captured as a HIP graph and
+    replayed for the later Krylov vectors (device work and host reads between
+    the replays) against the same solve with every matvec run eagerly: the
+    per-sweep energies and the final MPS are the same bits, and replays
+    happened."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(14)
+    L = 10
+    mpo, mps = _setup(dor.xxz_field_dense(0.6, rng.uniform(-1.0, 1.0, L)), seed=7)
+    opts = qa.DmrgOptions(cutoff=1e-10, convergence_criterion=0.0, maximum_bond=32, minimum_bond=4,
+                          maximum_iterations=3)
+    runs = []
+    try:
+        for graphs in (False, True):
+            ctx.set_graphs(graphs)
+            before = ctx.lanczos_stats()["graph_replays"]
+            with _lanczos_ctx(ctx, 24, 1e-12):
+                E, new, lg = qa.dmrg([to_dev(ctx, t) for t in mpo], [to_dev(ctx, t) for t in mps], opts)
+            runs.append((lg["energies"], [from_dev(t) for t in new], ctx.lanczos_stats()["graph_replays"] - before))
+    finally:
+        ctx.set_graphs(True)
+    (e0, m0, r0), (e1, m1, r1) = runs
+    assert r0 == 0 and r1 > 0
+    assert e0 == e1
+    for a, b in zip(m0, m1):
+        assert sorted(a.blocks) == sorted(b.blocks)
+        for k in a.blocks:
+            assert np.array_equal(a.blocks[k], b.blocks[k])

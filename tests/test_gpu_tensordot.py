@@ -6,7 +6,7 @@ split-FP32 mode."""
 import numpy as np
 import pytest
 
-from oracle.btensor_oracle import tensordot as ref_tensordot
+from oracle.btensor_oracle import BTensor, tensordot as ref_tensordot
 from tests.helpers import (assert_same_structure, assert_values_close, from_dev, inv_leg, rand_btensor, rand_leg,
                            to_dev)
 
@@ -182,3 +182,53 @@ def test_tensordot_incompatible_raises(ctx):
     a, b = to_dev(ctx, A), to_dev(ctx, B)
     with pytest.raises(qa.CheckError):
         a.tensordot(b, [0], [0])  # charges of these legs do not cancel
+
+
+def _variant_cases(rng):
+    from quantit_amd import workloads
+    X, A, dx, da, _ = workloads.contract_svd(256)
+    out = [(BTensor(**X), BTensor(**A), dx, da)]
+    for (ra, rb), da2, db2 in CASES:
+        a, b = _pair(rng, G1, ra, rb, da2, db2, nsec=4, smax=40 if ra + rb <= 7 else 6)
+        out.append((a, b, da2, db2))
+    # one long-K pair and one wide output block
+    big_a = rand_btensor(rng, (0,), [[(300, (0,))], [(700, (0,))]], (0,))
+    big_b = rand_btensor(rng, (0,), [[(700, (0,))], [(260, (0,))]], (0,))
+    out.append((big_a, big_b, [1], [0]))
+    return out
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+def test_gemm_tile_variants_give_the_same_bits(ctx, variant, monkeypatch):
+    """Every FP64 tile variant (64x64 double-buffered, 128x64, 64x128, 128x128)
+    against the 64x64 kernel: bit-identical blocks (same per-element
+    accumulation order)."""
+    rng = np.random.default_rng(40 + variant)
+    for a, b, da, db in _variant_cases(rng):
+        monkeypatch.setenv("QTT_GEMM_VARIANT", "0")
+        ref = from_dev(to_dev(ctx, a).tensordot(to_dev(ctx, b), da, db))
+        monkeypatch.setenv("QTT_GEMM_VARIANT", str(variant))
+        got = from_dev(to_dev(ctx, a).tensordot(to_dev(ctx, b), da, db))
+        assert sorted(got.blocks) == sorted(ref.blocks)
+        for k in ref.blocks:
+            assert np.array_equal(got.blocks[k], ref.blocks[k]), (variant, k)
+
+
+def test_gemm_tuning_fills_the_table_and_keeps_the_bits(ctx, monkeypatch, tmp_path):
+    import quantit_amd as qa
+    from quantit_amd import workloads
+    X, A, dx, da, _ = workloads.contract_svd(512)
+    x, a = qa.BTensor.from_parts(ctx, **X), qa.BTensor.from_parts(ctx, **A)
+    monkeypatch.setenv("QTT_GEMM_VARIANT", "0")
+    ref = from_dev(x.tensordot(a, dx, da))
+    monkeypatch.delenv("QTT_GEMM_VARIANT")
+    monkeypatch.setenv("QTT_GEMM_TUNE", "1")
+    n0 = qa.gemm_tuning_size()
+    first = from_dev(x.tensordot(a, dx, da))    # tunes this shape class
+    second = from_dev(x.tensordot(a, dx, da))   # uses the table
+    assert qa.gemm_tuning_size() >= n0
+    for got in (first, second):
+        for k in ref.blocks:
+            assert np.array_equal(got.blocks[k], ref.blocks[k])
+    qa.gemm_tuning_save(tmp_path / "tuning.json")
+    assert "gemm_variants" in (tmp_path / "tuning.json").read_text()
