@@ -336,22 +336,12 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
       if (tid == 0) nrot = 0;
       __syncthreads();
       for (int r = 0; r < nsteps; ++r) {
-        if (tid < NB2 / 2) {
-          int x, y;
-          pair_of(tid, r, x, y);
-          const double app = G[0][x][x], aqq = G[0][y][y], apq = G[0][x][y];
-          double c = 1.0, sn = 0.0;
-          if (app > 0.0 && aqq > 0.0 && apq * apq > (DBL_EPSILON * app) * (DBL_EPSILON * aqq)) {
-            jacobi_cs(aqq - app, 2.0 * apq, c, sn);
-            atomicAdd(&nrot, 1);
-          }
-          rc[tid] = c;
-          rs[tid] = sn;
-        }
         int a, a2, b, b2;
         pair_of(P, r, a, a2);
         pair_of(R, r, b, b2);
-        // only the block owners read (the other waves would multiply the LDS traffic)
+        // The block owners issue their reads first, so that their latency
+        // runs under the rotations below (only the owners read: the other
+        // waves would multiply the LDS traffic).
         double g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0, q00 = 0.0, q01 = 0.0, q10 = 0.0, q11 = 0.0;
         if (blk) {
           g00 = G[0][a][b];
@@ -362,6 +352,19 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
           q01 = Q[0][a][b2];
           q10 = Q[0][a2][b];
           q11 = Q[0][a2][b2];
+        }
+        if (tid < NB2 / 2) {
+          int x, y;
+          pair_of(tid, r, x, y);
+          const double app = G[0][x][x], aqq = G[0][y][y], apq = G[0][x][y];
+          // computed unconditionally and selected, so that the chain has no
+          // branch (an unselected result may be NaN, e.g. for a zero column)
+          const bool rot = app > 0.0 && aqq > 0.0 && apq * apq > (DBL_EPSILON * app) * (DBL_EPSILON * aqq);
+          double cc, ss;
+          jacobi_cs(aqq - app, 2.0 * apq, cc, ss);
+          rc[tid] = rot ? cc : 1.0;
+          rs[tid] = rot ? ss : 0.0;
+          if (rot) nrot = 1;  // a flag: every writer stores the same value
         }
         __syncthreads();  // rotations published; every read of this step precedes the writes
         if (blk) {

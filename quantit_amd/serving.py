@@ -58,16 +58,21 @@ class Result:
         return self.finish - self.arrival
 
 
-def footprint_bytes(L, D, w=MPO_BOND, d=2):
+def footprint_bytes(L, D, w=MPO_BOND, d=2, krylov=0):
     """HBM bytes a solve holds at bond dimension D.
 
     Per site: the MPS tensor (D d D), the left and right environments
     (D w D each) and the two-site Hamiltonian (w d^2 w d^2, negligible).
     Per update: theta and the Lanczos vectors (4 of D d d D), the SVD work
     area (Y, ping-pong copy, W for the (D d) x (d D) matrix) and the
-    contraction intermediates (L.theta: D w d d D, twice)."""
+    contraction intermediates (L.theta: D w d d D, twice).  With the k-step
+    Lanczos eigensolver (krylov >= 2) also the Krylov basis (krylov vectors
+    of D d d D) and the captured matvec's own input, intermediates and
+    output (D d d D + 2 of D w d d D)."""
     site = 8.0 * (D * d * D + 2 * D * w * D)
     update = 8.0 * (4 * D * d * d * D + 3 * (d * D) ** 2 + 2 * D * w * d * d * D)
+    if krylov >= 2:
+        update += 8.0 * ((krylov + 1) * D * d * d * D + 2 * D * w * d * d * D)
     return int(ARENA_SLACK * (L * site + update)) + (64 << 20)
 
 
@@ -150,7 +155,7 @@ class Server:
                     state["next"] += 1
                 with self.cv:
                     for r in list(waiting):
-                        need = footprint_bytes(r.L, r.D)
+                        need = footprint_bytes(r.L, r.D, krylov=self.max_krylov)
                         if need > self.budget:
                             waiting.remove(r)
                             results[r.rid] = MemoryError(f"request {r.rid} needs {need} bytes")
