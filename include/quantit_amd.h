@@ -167,6 +167,27 @@ int qtt_eigh(const qtt_btensor* t, size_t split, qtt_btensor** d, qtt_btensor** 
 /* truncate(std::tuple<U, d, V>&&, max, min, tol, pow) (LinearAlgebra.h:244-247) */
 int qtt_truncate(const qtt_btensor* U, const qtt_btensor* d, const qtt_btensor* V, size_t max_size, size_t min_size,
                  double tol, double pow, qtt_btensor** U_out, qtt_btensor** d_out, qtt_btensor** V_out);
+/* svd(const btensor&, BOOL some, BOOL compute_uv) of a rank-2 tensor
+ * (LinearAlgebra.h:74, sources/btensor_linalg.cpp:390-502).  some = 0 asks for
+ * full square unitaries; the reference then sizes d, U and V by max(m, n) per
+ * sector while the sector factors have min(m, n) values, and compute_uv = 0
+ * (torch::svd's m x m and n x n zero matrices) is no better, so in those
+ * modes only sectors with m = n are accepted (there every form has the
+ * economy structure) and a rectangular sector gives QTT_INVALID_ARGUMENT.
+ * compute_uv = 0 returns U and V filled with zeros. */
+int qtt_svd_matrix(const qtt_btensor* t, int some, int compute_uv, qtt_btensor** U, qtt_btensor** d,
+                   qtt_btensor** V);
+/* eigh(const btensor&, size_t split, tol, min_size, max_size, pow)
+ * (LinearAlgebra.h:180-181, sources/btensor_linalg.cpp:816-823): eigh then
+ * the eigen truncate below.  The reference's default pow here is 1. */
+int qtt_eigh_truncated(const qtt_btensor* t, size_t split, double tol, size_t min_size, size_t max_size, double pow,
+                       qtt_btensor** d, qtt_btensor** U);
+/* truncate(btensor&& e, btensor&& S, max, min, tol, pow) (LinearAlgebra.h:242-243,
+ * sources/btensor_linalg.cpp:786-805): truncate_impl with the eigenvalues as
+ * weights and the eigenvectors S as the one unitary.  Eigenvalues are
+ * ascending within a sector, so a sector is kept whole or erased whole. */
+int qtt_truncate_eigh(const qtt_btensor* e, const qtt_btensor* S, size_t max_size, size_t min_size, double tol,
+                      double pow, qtt_btensor** e_out, qtt_btensor** S_out);
 
 /* ---- DMRG ----------------------------------------------------------------
  * dmrg(bMPO&, bMPS&, const dmrg_options&, dmrg_logger&) (include/dmrg.h:39-40,

@@ -111,6 +111,9 @@ SIGNATURES = {
     "qtt_svd_truncated": (C.c_int, [P, C.c_size_t, C.c_double, C.c_size_t, C.c_size_t, C.c_double, PP, PP, PP]),
     "qtt_eigh": (C.c_int, [P, C.c_size_t, PP, PP]),
     "qtt_truncate": (C.c_int, [P, P, P, C.c_size_t, C.c_size_t, C.c_double, C.c_double, PP, PP, PP]),
+    "qtt_svd_matrix": (C.c_int, [P, C.c_int, C.c_int, PP, PP, PP]),
+    "qtt_eigh_truncated": (C.c_int, [P, C.c_size_t, C.c_double, C.c_size_t, C.c_size_t, C.c_double, PP, PP]),
+    "qtt_truncate_eigh": (C.c_int, [P, P, C.c_size_t, C.c_size_t, C.c_double, C.c_double, PP, PP]),
     "qtt_dmrg": (C.c_int, [C.c_int, PP, PP, C.POINTER(DmrgOptionsC), DP, C.c_int, DP, DP, I64P, C.POINTER(C.c_int),
                            DP, DP]),
     "qtt_hamil2site_times_state": (C.c_int, [P, P, P, P, PP]),
@@ -413,12 +416,34 @@ def svd(t, split, tol=None, min_size=1, max_size=2 ** 64 - 1, pow=2.0):
                   float(pow))
 
 
-def eigh(t, split):
+def svd_matrix(t, some=True, compute_uv=True):
+    """``svd(btensor, some, compute_uv)`` of a rank-2 tensor
+    (include/blockTensor/LinearAlgebra.h:74).  ``some=False`` and
+    ``compute_uv=False`` accept square sectors only (ValueError otherwise);
+    ``compute_uv=False`` returns zero-filled U and V."""
+    return _three(t.ctx, library().qtt_svd_matrix, t.h, int(bool(some)), int(bool(compute_uv)))
+
+
+def eigh(t, split, tol=None, min_size=1, max_size=2 ** 64 - 1, pow=1.0):
     """``eigh(btensor, split)`` (include/blockTensor/LinearAlgebra.h:167):
-    (eigenvalues ascending per sector, U)."""
+    (eigenvalues ascending per sector, U); with ``tol`` the truncating
+    ``eigh(A, split, tol, min_size, max_size, pow=1)`` (:180-181)."""
     hs = [C.c_void_p(), C.c_void_p()]
-    _check(library().qtt_eigh(t.h, int(split), C.byref(hs[0]), C.byref(hs[1])))
+    if tol is None:
+        _check(library().qtt_eigh(t.h, int(split), C.byref(hs[0]), C.byref(hs[1])))
+    else:
+        _check(library().qtt_eigh_truncated(t.h, int(split), float(tol), int(min_size), int(max_size), float(pow),
+                                            C.byref(hs[0]), C.byref(hs[1])))
     return BTensor(t.ctx, hs[0]), BTensor(t.ctx, hs[1])
+
+
+def truncate_eigh(e, S, max_size, min_size, tol, pow=1.0):
+    """``truncate(btensor&& e, btensor&& S, max, min, tol, pow)``
+    (include/blockTensor/LinearAlgebra.h:242-243): (e, S) truncated together."""
+    hs = [C.c_void_p(), C.c_void_p()]
+    _check(library().qtt_truncate_eigh(e.h, S.h, int(max_size), int(min_size), float(tol), float(pow),
+                                       C.byref(hs[0]), C.byref(hs[1])))
+    return BTensor(e.ctx, hs[0]), BTensor(e.ctx, hs[1])
 
 
 def truncate(U, d, V, max_size, min_size, tol, pow=2.0):
@@ -440,11 +465,28 @@ class DmrgOptions:
         self.maximum_iterations = maximum_iterations
 
 
-def dmrg(mpo, mps, options, oc=0):
+def dmrg(mpo, mps, options, oc=0, seed=0):
     """``dmrg(bMPO&, bMPS&, options, logger)`` (include/dmrg.h:39-40) on a
     state whose orthogonality centre is at site ``oc`` (moved back there at
     the end, as the reference does).  Returns (energy, new mps list, log
-    dict)."""
+    dict).
+
+    With a charge (an int, or a tuple of ints per charge group) in place of
+    ``mps``: ``dmrg(bMPO&, qnum, options, logger)`` (sources/dmrg.cpp:109-118),
+    which starts from ``random_bMPS(options.minimum_bond, hamil, qnum)``: a
+    random state in that sector on the MPO's physical legs
+    (``models.random_bmps_general`` seeded with ``seed``), optimised from
+    site 0 without a canonical preparation, as the reference does."""
+    if isinstance(mps, (int, np.integer)) or (isinstance(mps, tuple) and all(
+            isinstance(c, (int, np.integer)) for c in mps)):
+        from . import models
+        phys, groups = models.mpo_phys_legs(mpo)
+        q = (int(mps),) if isinstance(mps, (int, np.integer)) else tuple(int(c) for c in mps)
+        if len(q) != len(groups):
+            raise ValueError(f"charge {q} does not match the MPO's {len(groups)} charge groups")
+        parts = models.random_bmps_general(int(options.minimum_bond), phys, q, seed, groups)
+        mps = [BTensor.from_parts(mpo[0].ctx, **p) for p in parts]
+        oc = 0
     L = len(mpo)
     mpo_h = (C.c_void_p * L)(*[t.h for t in mpo])
     mps_h = (C.c_void_p * L)(*[t.h for t in mps])

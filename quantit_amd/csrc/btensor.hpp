@@ -138,6 +138,9 @@ struct EighResult {
 };
 EighResult eigh(const BTensor& t, size_t split);
 SvdResult truncate(SvdResult&& udv, size_t max_size, size_t min_size, double tol, double pow);
+EighResult truncate(EighResult&& e_s, size_t max_size, size_t min_size, double tol, double pow);
+EighResult eigh_truncated(const BTensor& t, size_t split, double tol, size_t min_size, size_t max_size, double pow);
+SvdResult svd_matrix(const BTensor& t, bool some, bool compute_uv);
 // Normalises d in place on device: d /= sqrt(sum(d^2)).
 void normalize_d(BTensor& d);
 

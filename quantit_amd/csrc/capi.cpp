@@ -520,6 +520,56 @@ int qtt_truncate(const qtt_btensor* U, const qtt_btensor* d, const qtt_btensor* 
   });
 }
 
+int qtt_eigh_truncated(const qtt_btensor* t, size_t split, double tol, size_t min_size, size_t max_size, double pow,
+                       qtt_btensor** d, qtt_btensor** U) {
+  return guard([&] {
+    const qtt::BTensor& tt = live(t, "tensor");
+    need(U, "U");
+    need(d, "d");
+    auto r = qtt::eigh_truncated(tt, split, tol, min_size, max_size, pow);
+    *d = wrap(std::move(r.d));
+    *U = wrap(std::move(r.U));
+  });
+}
+
+int qtt_truncate_eigh(const qtt_btensor* e, const qtt_btensor* S, size_t max_size, size_t min_size, double tol,
+                      double pow, qtt_btensor** e_out, qtt_btensor** S_out) {
+  return guard([&] {
+    const qtt::BTensor& te = live(e, "e");
+    const qtt::BTensor& ts = live(S, "S");
+    need(e_out, "e_out");
+    need(S_out, "S_out");
+    same_context(te, ts);
+    if (te.rank() != 1) throw qtt::InvalidArgument("truncate: e must have rank 1");
+    if (ts.rank() < 1) throw qtt::InvalidArgument("truncate: S needs at least one dimension");
+    const int nd = te.shape.nsec(0);
+    if (ts.shape.nsec(ts.rank() - 1) != nd)
+      throw qtt::InvalidArgument("truncate: the last dimension of S must have the sections of e");
+    for (auto& b : te.blocks)
+      if (ts.shape.secs[ts.rank() - 1][b.idx.i[0]] != te.shape.secs[0][b.idx.i[0]])
+        throw qtt::InvalidArgument("truncate: section sizes of e and S differ");
+    qtt::EighResult in{te, ts};
+    auto r = qtt::truncate(std::move(in), max_size, min_size, tol, pow);
+    *e_out = wrap(std::move(r.d));
+    *S_out = wrap(std::move(r.U));
+  });
+}
+
+int qtt_svd_matrix(const qtt_btensor* t, int some, int compute_uv, qtt_btensor** U, qtt_btensor** d,
+                   qtt_btensor** V) {
+  return guard([&] {
+    const qtt::BTensor& tt = live(t, "tensor");
+    need(U, "U");
+    need(d, "d");
+    need(V, "V");
+    if (tt.rank() != 2) throw qtt::InvalidArgument("svd(tensor, some, compute_uv): the tensor must have rank 2");
+    auto r = qtt::svd_matrix(tt, some != 0, compute_uv != 0);
+    *U = wrap(std::move(r.U));
+    *d = wrap(std::move(r.d));
+    *V = wrap(std::move(r.V));
+  });
+}
+
 namespace {
 
 // Tensors of an MPS/MPO list, all of one context.

@@ -919,15 +919,20 @@ void remove_unit_blocks(std::vector<Block>& lst, int rank, int kd) {
 
 }  // namespace
 
+namespace {
+
 // truncate_impl (sources/btensor_linalg.cpp:657-755) with the cut computed on
-// device; U and V keep their block order, sliced blocks are compacted into
-// fresh storage.
-SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol, double pw) {
-  BTensor& d = in.d;
+// device, for d and any number of unitaries whose last leg carries d's
+// sections (U and V of an svd, the eigenvectors of an eigh).  The unitaries
+// keep their block order, sliced blocks are compacted into fresh storage;
+// *units are replaced and the truncated d is returned.
+BTensor truncate_units(BTensor&& d_in, const std::vector<BTensor*>& units, size_t max_size, size_t min_size,
+                       double tol, double pw) {
+  BTensor d = std::move(d_in);
   Context* ctx = d.ctx;
   double* ab = arena_base(ctx);
   const int nd = int(d.blocks.size());
-  if (nd == 0) return std::move(in);
+  if (nd == 0) return d;
   std::vector<int64_t> s_offs(nd);
   std::vector<int32_t> ks(nd);
   int64_t total = 0;
@@ -936,7 +941,7 @@ SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol,
     ks[i] = int32_t(d.blocks[i].numel);
     total += ks[i];
   }
-  if (total == 0) return std::move(in);
+  if (total == 0) return d;
   int64_t n2 = 1;
   while (n2 < total) n2 <<= 1;
   BufPtr scratch = ctx->alloc(size_t(n2 + 1) * sizeof(double) + size_t(nd) * sizeof(int32_t) + 256);
@@ -953,24 +958,24 @@ SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol,
   std::vector<int32_t> kept(nd);
   ctx->read_small(kept.data(), d_kept, size_t(nd) * sizeof(int32_t));
 
-  BTensor& U = in.U;
-  BTensor& V = in.V;
-  const int ru = U.rank(), rv = V.rank();
-  std::vector<Block> ub = U.blocks, vb = V.blocks, db = d.blocks;
-  std::vector<int> slice_to(nd, -1);  // per d section: new column count, -1 = unchanged
-  Shape dsh = d.shape, ush = U.shape, vsh = V.shape;
+  const size_t nu = units.size();
+  std::vector<std::vector<Block>> ub(nu);
+  std::vector<Shape> ush(nu);
+  for (size_t u = 0; u < nu; ++u) {
+    ub[u] = units[u]->blocks;
+    ush[u] = units[u]->shape;
+  }
+  std::vector<Block> db = d.blocks;
+  Shape dsh = d.shape;
   std::vector<char> d_erased(nd, 0);
   for (int i = nd - 1; i >= 0; --i) {
     const int kd = db[i].idx.i[0];
     if (kept[i] == 0) {
-      remove_unit_blocks(ub, ru, kd);
-      remove_unit_blocks(vb, rv, kd);
+      for (size_t u = 0; u < nu; ++u) remove_unit_blocks(ub[u], units[u]->rank(), kd);
       d_erased[i] = 1;
     } else {
       dsh.secs[0][kd] = kept[i];
-      ush.secs[ru - 1][kd] = kept[i];
-      vsh.secs[rv - 1][kd] = kept[i];
-      slice_to[kd] = kept[i];
+      for (size_t u = 0; u < nu; ++u) ush[u].secs[units[u]->rank() - 1][kd] = kept[i];
     }
   }
   auto rebuild = [&](const BTensor& src, const std::vector<Block>& keep, const Shape& nsh, int r) {
@@ -1022,11 +1027,77 @@ SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol,
   std::vector<Block> dkeep;
   for (int i = 0; i < nd; ++i)
     if (!d_erased[i]) dkeep.push_back(db[i]);
-  SvdResult out;
-  out.U = rebuild(U, ub, ush, ru);
-  out.V = rebuild(V, vb, vsh, rv);
-  out.d = rebuild(d, dkeep, dsh, 1);
+  for (size_t u = 0; u < nu; ++u) *units[u] = rebuild(*units[u], ub[u], ush[u], units[u]->rank());
+  return rebuild(d, dkeep, dsh, 1);
+}
+
+}  // namespace
+
+SvdResult truncate(SvdResult&& in, size_t max_size, size_t min_size, double tol, double pw) {
+  SvdResult out = std::move(in);
+  out.d = truncate_units(std::move(out.d), {&out.U, &out.V}, max_size, min_size, tol, pw);
   return out;
+}
+
+// truncate(std::tuple<btensor, btensor>&& e_S, ...) (sources/btensor_linalg.cpp:786-800):
+// the eigenvalues take the place of the singular values and the eigenvectors
+// are truncated with them.  (The reference passes the eigenvalue tensor a
+// second time where the eigenvectors belong, btensor_linalg.cpp:797, so its
+// second output is a moved-from tensor; the eigenvectors are what the
+// declaration and the truncating eigh above it mean.)  Eigenvalues come in
+// ascending order within a sector, and a sector keeps the values before the
+// first one not above the cut (lower_bound_impl2, btensor_linalg.cpp:545-555):
+// a sector whose smallest eigenvalue is at or below the cut is erased whole.
+EighResult truncate(EighResult&& in, size_t max_size, size_t min_size, double tol, double pw) {
+  EighResult out = std::move(in);
+  out.d = truncate_units(std::move(out.d), {&out.U}, max_size, min_size, tol, pw);
+  return out;
+}
+
+EighResult eigh_truncated(const BTensor& t, size_t split, double tol, size_t min_size, size_t max_size, double pw) {
+  return truncate(eigh(t, split), max_size, min_size, tol, pw);
+}
+
+// svd(const btensor&, BOOL some, BOOL compute_uv) (sources/btensor_linalg.cpp:390-502)
+// of a rank-2 tensor.  With some or compute_uv false the reference sizes the
+// unitaries' last sections by max(m, n) (or torch::svd returns m x m and
+// n x n zero matrices) while d keeps min(m, n) values per sector; the
+// structure is consistent only for square sectors, where the result equals
+// the economy form.  Rectangular sectors are rejected in those modes.
+SvdResult svd_matrix(const BTensor& t, bool some, bool compute_uv) {
+  if (t.rank() != 2) throw InvalidArgument("svd(tensor, some, compute_uv) takes a rank-2 tensor");
+  if (!some || !compute_uv) {
+    for (const auto& p : compact_dense_plan(t))
+      if (p.m != p.n)
+        throw InvalidArgument("svd with some = false or compute_uv = false: sector of " + std::to_string(p.m) + " x " +
+                              std::to_string(p.n) + " is not square (the reference's d, U and V sizes disagree there)");
+  }
+  SvdResult r = svd(t, 1);
+  if (!compute_uv) {  // zero-filled U and V of the same structure
+    Context* ctx = t.ctx;
+    double* ab = arena_base(ctx);
+    std::vector<kern::Copy2D> z;
+    std::vector<kern::ChunkTask> zt;
+    for (BTensor* u : {&r.U, &r.V}) {
+      for (const auto& b : u->blocks) {
+        if (!b.numel) continue;
+        kern::Copy2D c{};
+        c.src_off = -1;
+        c.dst_off = u->ptr(b) - ab;
+        c.rows = 1;
+        c.cols = int32_t(b.numel);
+        c.dst_cs = 1;
+        add_chunks(zt, int(z.size()), b.numel);
+        z.push_back(c);
+      }
+    }
+    if (!zt.empty()) {
+      KernelTimer kt(*ctx, "svd_zero_uv");
+      auto [dz, dzt] = ctx->upload_pack(z, zt);
+      kern::copy2d(ctx->stream(), ab, dz, dzt, int(zt.size()));
+    }
+  }
+  return r;
 }
 
 SvdResult svd_truncated(const BTensor& t, size_t split, double tol, size_t min_size, size_t max_size, double pw,

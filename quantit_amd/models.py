@@ -158,6 +158,17 @@ def random_bmps(ctx, L, bond, q=0, seed=0, phys=UP_FIRST):
     return [BTensor.from_parts(ctx, **p) for p in random_bmps_parts(L, bond, q, seed, phys)]
 
 
+def mpo_phys_legs(mpo):
+    """Physical legs of the states an MPO acts on: the inverse of each
+    site's last leg, ``hamil[i].shape_from({0,0,0,-1}).inverse_cvals()``
+    (sources/MPT.cpp:480-488), as lists of (section size, charge)."""
+    out = []
+    for W in mpo:
+        groups, secs, cvals, _ = W.structure()
+        out.append([(int(n), _q_inv(groups, tuple(c))) for n, c in zip(secs[3], cvals[3])])
+    return out, tuple(groups)
+
+
 def bmpo_parts(dense, phys=UP_FIRST, cutoff=1e-14):
     """Block structure and data of a U(1) bMPO from dense MPO tensors (l, s, r, s').
     Returns one dict (groups, secs, cvals, sel, blocks) per site.
@@ -212,47 +223,104 @@ def bmpo_parts(dense, phys=UP_FIRST, cutoff=1e-14):
     return out
 
 
-def random_bmps_parts(L, bond, q=0, seed=0, phys=UP_FIRST):
-    """Random U(1) bMPS of L spin-1/2 sites in the sector of total 2Sz = q.
+def _q_add(groups, a, b):
+    """Sum of two charges: Z components add, C<N> components add mod N
+    (stored in [0, N)); int16 wrap-around as the stored values."""
+    out = []
+    for g, x, y in zip(groups, a, b):
+        v = x + y
+        v = v % g if g else ((v + 32768) % 65536) - 32768
+        out.append(int(v))
+    return tuple(out)
 
-    Structure as ``random_bMPS`` (sources/MPT.cpp:311-494): ``bond`` random
-    physical strings, each repaired in one left-to-right pass toward total
-    charge q; the right bond of site i has one section per distinct partial
-    charge q - sum_{t<=i} c(s_t), sized by its multiplicity; site 0 carries the
-    selection rule q; the last right bond keeps one element.  Blocks are
-    uniform in [0, 1), each site normalised.  Seeded with numpy's PCG64."""
+
+def _q_inv(groups, a):
+    return tuple(int((-x) % g) if g else int(((-x + 32768) % 65536) - 32768) for g, x in zip(groups, a))
+
+
+def _q_dist2(groups, a, b):
+    """Squared distance between charges (the repair metric of
+    generate_random_string): the difference per component, taken as the
+    shorter way round for C<N>."""
+    t = 0
+    for g, x, y in zip(groups, a, b):
+        d = x - y
+        if g:
+            d %= g
+            d = min(d, g - d)
+        t += d * d
+    return t
+
+
+def random_bmps_general(bond, phys, q, seed=0, groups=(0,)):
+    """Random bMPS structure and blocks of ``random_bMPS_impl``
+    (sources/MPT.cpp:423-462) for any charge groups and physical legs.
+
+    phys[i] is site i's physical leg as a list of (section size, charge),
+    i.e. the inverse of the MPO's last leg (``random_bMPS(bond, hamil, q)``,
+    sources/MPT.cpp:478-493).  ``bond`` strings of physical sections are drawn
+    uniformly and repaired in one pass toward total charge q
+    (``generate_random_string``, :311-371: a site's section is replaced only
+    by one strictly closer to q); the right bond of site i has one section
+    per distinct partial charge q - sum_{t<=i} c(s_t), sized by its
+    multiplicity (``make_right_side``, :387-421); site 0 carries the selection
+    rule q, the others the neutral one; the last right bond keeps one element.
+    Blocks are uniform in [0, 1) from numpy's PCG64 seeded with ``seed``, each
+    site normalised."""
+    L = len(phys)
+    q = tuple(q)
+    neutral = tuple(0 for _ in groups)
     rng = np.random.default_rng(seed)
-    ps = [c[0] for c in phys]
-    strings = rng.integers(0, 2, size=(bond, L))
+    counts = [len(p) for p in phys]
+    if len(set(counts)) == 1:
+        strings = rng.integers(0, counts[0], size=(bond, L))
+    else:
+        strings = np.stack([rng.integers(0, n, size=bond) for n in counts], axis=1)
+    pc = [[c for _, c in p] for p in phys]
     for j in range(bond):
-        total = sum(ps[s] for s in strings[j])
+        total = neutral
         for i in range(L):
-            if total == q:
+            total = _q_add(groups, total, pc[i][strings[j, i]])
+        cur = _q_dist2(groups, total, q)
+        for i in range(L):
+            if cur == 0:
                 break
-            best = min(range(2), key=lambda s: abs(total - ps[strings[j, i]] + ps[s] - q))
-            total += ps[best] - ps[strings[j, i]]
+            rest = _q_add(groups, total, _q_inv(groups, pc[i][strings[j, i]]))
+            best = int(strings[j, i])
+            for s_ in range(counts[i]):
+                nd = _q_dist2(groups, _q_add(groups, rest, pc[i][s_]), q)
+                if nd < cur:
+                    best, cur = s_, nd
             strings[j, i] = best
-        if total != q:
-            raise ValueError("charge sector unreachable")
-    acc = np.full(bond, q, dtype=np.int64)
-    lsecs, lcv = [1], [0]
+            total = _q_add(groups, rest, pc[i][best])
+        if cur != 0:
+            raise ValueError("charge sector unreachable with these physical legs")
+    acc = [q] * bond
+    lsecs, lcv = [1], [neutral]
     out = []
     for i in range(L):
-        acc = acc - np.array([ps[s] for s in strings[:, i]])
-        rsecs, rcv = _charge_runs(sorted(acc.tolist()))
+        acc = [_q_add(groups, a, _q_inv(groups, pc[i][strings[j, i]])) for j, a in enumerate(acc)]
+        rsecs, rcv = _charge_runs(sorted(acc))
         if i == L - 1:
             rsecs, rcv = [1], rcv[:1]
-        sel = q if i == 0 else 0
+        sel = q if i == 0 else neutral
+        psz = [n for n, _ in phys[i]]
         blocks = {}
         for a, ca in enumerate(lcv):
-            for s in range(2):
+            for s_ in range(counts[i]):
                 for b, cb in enumerate(rcv):
-                    if ca + ps[s] + cb == sel:
-                        blocks[(a, s, b)] = rng.random((lsecs[a], 1, rsecs[b]))
+                    if _q_add(groups, _q_add(groups, ca, pc[i][s_]), cb) == sel:
+                        blocks[(a, s_, b)] = rng.random((lsecs[a], psz[s_], rsecs[b]))
         nrm = np.sqrt(sum(float(np.sum(x * x)) for x in blocks.values()))
         blocks = {k: v / nrm for k, v in blocks.items()}
-        out.append(dict(groups=(0,), secs=[lsecs, [1, 1], rsecs],
-                        cvals=[[(c,) for c in lcv], [(p,) for p in ps], [(c,) for c in rcv]], sel=(sel,),
-                        blocks=blocks))
-        lsecs, lcv = rsecs, [-c for c in rcv]
+        out.append(dict(groups=tuple(groups), secs=[lsecs, psz, rsecs], cvals=[list(lcv), list(pc[i]), list(rcv)],
+                        sel=sel, blocks=blocks))
+        lsecs, lcv = rsecs, [_q_inv(groups, c) for c in rcv]
     return out
+
+
+def random_bmps_parts(L, bond, q=0, seed=0, phys=UP_FIRST):
+    """Random U(1) bMPS of L spin-1/2 sites in the sector of total 2Sz = q:
+    ``random_bmps_general`` with the physical leg ``phys`` (one state per
+    section) on every site."""
+    return random_bmps_general(bond, [[(1, c) for c in phys]] * L, (q,), seed, (0,))

@@ -259,3 +259,95 @@ def test_svd_one_group_widths(ctx, m, n, decades, zeros):
     for t in (U, V):
         M = t.to_dense()
         assert np.abs(M.T @ M - np.eye(M.shape[1])).max() < (1e-10 if decades and decades > 8 else 1e-12)
+
+
+def _sym_case(seed):
+    rng = np.random.default_rng(seed)
+    rows = [(5, (0,)), (3, (2,)), (7, (-2,)), (9, (4,)), (2, (-4,))]
+    A = rand_btensor(rng, G1, [rows, inv_leg(G1, rows)], (0,))
+    for k in A.blocks:
+        A.blocks[k] = A.blocks[k] + A.blocks[k].T
+    return A
+
+
+@pytest.mark.parametrize("tol,mn,mx,pw", [(1e-1, 1, 10 ** 6, 1.0), (2.0, 1, 10 ** 6, 1.0), (0.5, 4, 12, 2.0),
+                                          (1e-8, 1, 6, 1.0), (5.0, 20, 10 ** 6, 1.0)])
+def test_eigh_truncated_matches_oracle(ctx, tol, mn, mx, pw):
+    """Truncating eigh (btensor_linalg.cpp:816-823) against the oracle's
+    truncate_impl of its eigh: structure bit-exact (erased sectors included),
+    kept eigenvalues within 1e-12 of the norm, A.U = U.Lambda on what is
+    kept; the 2-tuple truncate gives the same bits as the truncating eigh."""
+    import quantit_amd as qa
+    from oracle.btensor_oracle import truncate_impl
+    A = _sym_case(43)
+    re, rS = eigh_split(A, 1)
+    re2, (rS2,), _ = truncate_impl(re, [rS], mx, mn, tol, pw)
+    dA = to_dev(ctx, A)
+    e2, S2 = qa.eigh(dA, 1, tol=tol, min_size=mn, max_size=mx, pow=pw)
+    he, hS = from_dev(e2), from_dev(S2)
+    assert_same_structure(he, re2)
+    assert_same_structure(hS, rS2)
+    nrm = np.linalg.norm(A.to_dense(), 2)
+    for k in re2.blocks:
+        assert np.abs(he.blocks[k] - re2.blocks[k]).max() <= 1e-12 * nrm
+    e, S = qa.eigh(dA, 1)
+    t2 = qa.truncate_eigh(e, S, mx, mn, tol, pw)
+    assert from_dev(t2[0]).blocks.keys() == he.blocks.keys()
+    for k in he.blocks:
+        assert np.array_equal(from_dev(t2[0]).blocks[k], he.blocks[k])
+    for k in hS.blocks:
+        assert np.array_equal(from_dev(t2[1]).blocks[k], hS.blocks[k])
+    # eigenpairs of what is kept
+    Ad = A.to_dense()
+    for (kd,), w in he.blocks.items():
+        cols = [ui for ui in hS.blocks if ui[-1] == kd]
+        for ui in cols:
+            r0 = sum(A.secs[0][:ui[0]])
+            blk = hS.blocks[ui]
+            Arow = Ad[r0:r0 + blk.shape[0]][:, r0:r0 + blk.shape[0]]
+            assert np.abs(Arow @ blk - blk * w).max() <= 1e-11 * nrm
+
+
+def test_eigh_truncated_known_answer(ctx):
+    """The hand-traced case of the oracle tests on the device: the sector
+    {-1, 4} is erased whole, {3, 5} and {0.5} are kept whole."""
+    import quantit_amd as qa
+    from tests.test_oracle_btensor import eigh_truncation_case
+    A = eigh_truncation_case()
+    e2, S2 = qa.eigh(to_dev(ctx, A), 1, tol=1.2, min_size=1, max_size=10, pow=1.0)
+    he, hS = from_dev(e2), from_dev(S2)
+    kept = sorted(np.round(np.concatenate([he.blocks[k] for k in he.blocks]), 12).tolist())
+    assert kept == [0.5, 3.0, 5.0]
+    assert len(he.blocks) == 2 and sorted(he.secs[0]) == [1, 2, 2]
+
+
+def test_svd_matrix_modes(ctx):
+    """svd(tensor, some, compute_uv) of a rank-2 tensor: some = True is the
+    split-1 svd bit for bit; on square sectors some = False gives the same;
+    compute_uv = False gives the same d and zero U, V; rectangular sectors
+    are rejected in those modes with ValueError."""
+    import quantit_amd as qa
+    A = _sym_case(44)     # square sectors
+    dA = to_dev(ctx, A)
+    ref = [from_dev(x) for x in qa.svd(dA, 1)]
+    for some, cuv in ((True, True), (False, True), (True, False), (False, False)):
+        got = [from_dev(x) for x in qa.svd_matrix(dA, some=some, compute_uv=cuv)]
+        for g, r in zip(got, ref):
+            assert_same_structure(g, r)
+        for k in ref[1].blocks:
+            assert np.array_equal(got[1].blocks[k], ref[1].blocks[k])
+        for gi, ri in ((got[0], ref[0]), (got[2], ref[2])):
+            for k in ri.blocks:
+                if cuv:
+                    assert np.array_equal(gi.blocks[k], ri.blocks[k])
+                else:
+                    assert not np.any(gi.blocks[k])
+    rng = np.random.default_rng(45)
+    rows = [(5, (0,)), (3, (1,))]
+    cols = [(4, (0,)), (3, (-1,))]
+    R = to_dev(ctx, rand_btensor(rng, G1, [rows, cols], (0,)))
+    assert len(from_dev(qa.svd_matrix(R)[1]).blocks) == 2
+    with pytest.raises(ValueError):
+        qa.svd_matrix(R, some=False)
+    with pytest.raises(ValueError):
+        qa.svd_matrix(R, compute_uv=False)

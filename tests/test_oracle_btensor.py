@@ -292,3 +292,48 @@ def test_truncate_erasures_follow_the_unsorted_list_known_answer():
     assert sorted(d2.blocks) == [(0,), (1,)]
     assert list(U2.blocks) == [(0, 3)]
 
+
+
+def eigh_truncation_case():
+    """Symmetric U(1) matrix with three sectors of known spectra: charge 0
+    {-1, 4}, charge 2 {3, 5}, charge -2 {0.5} (rotated by fixed orthogonal
+    matrices so that the blocks are not diagonal).  The first column section
+    has charge 0: the reference's eigenvector tensor takes the inverse of
+    that charge into its selection rule (btensor_linalg.cpp:294-330)."""
+    c, s = np.cos(0.3), np.sin(0.3)
+    R = np.array([[c, -s], [s, c]])
+    rows = [(2, (0,)), (1, (-2,)), (2, (2,))]
+    A = BTensor.from_sections((0,), [rows, [(n, (-q[0],)) for n, q in rows]], (0,))
+    A.blocks[(0, 0)] = R @ np.diag([-1.0, 4.0]) @ R.T
+    A.blocks[(1, 1)] = np.array([[0.5]])
+    A.blocks[(2, 2)] = R.T @ np.diag([3.0, 5.0]) @ R
+    return A
+
+
+def _by_charge(e, S):
+    """eigenvalue blocks keyed by the row charge of their sector"""
+    return {S.cvals[-1][k[0]][0]: e.blocks[k] for k in e.blocks}
+
+
+def test_eigh_truncation_erases_whole_sectors_known_answer():
+    """truncate(e, S, ...) of an eigh (btensor_linalg.cpp:786-823) traced by
+    hand: descending values 5, 4, 3, 0.5, -1; with tol 1.2 and pow 1 the
+    tail sums give |-1| = 1 <= 1.2 and 1 + 0.5 > 1.2, so the cut is the value
+    0.5 (minus 2 eps).  Within a sector the eigenvalues ascend and a sector
+    keeps the values before the first one not above the cut: the sector
+    {-1, 4} starts with -1 and is erased whole (its 4 included), {3, 5} and
+    {0.5} are kept whole."""
+    A = eigh_truncation_case()
+    e, S = eigh_split(A, 1)
+    full = _by_charge(e, S)
+    assert sorted(np.round(np.concatenate(list(full.values())), 12).tolist()) == [-1.0, 0.5, 3.0, 4.0, 5.0]
+    assert compute_last_index(np.array([5.0, 4.0, 3.0, 0.5, -1.0]), 1.2, 1.0, 1, 10) == 3
+    e2, (S2,), thr = truncate_impl(e, [S], 10, 1, 1.2, 1.0)
+    assert thr == 0.5 - 2 * 0.5 * np.finfo(float).eps
+    kept = _by_charge(e2, S2)
+    assert sorted(kept) == sorted(q for q, w in full.items() if w.min() > thr)
+    assert len(kept) == 2 and all(np.array_equal(kept[q], full[q]) for q in kept)
+    erased = [k for k in e.blocks if k not in e2.blocks]
+    assert len(erased) == 1 and e2.secs[0][erased[0][0]] == 2   # size left as it was
+    assert np.allclose(sorted(full[S.cvals[-1][erased[0][0]][0]]), [-1.0, 4.0])
+    assert all(k[-1] != erased[0][0] for k in S2.blocks)
