@@ -158,22 +158,30 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
   const int cross = env_int("QTT_SVD_CROSS_INNER", 1);
   // enqueues sweep `sweep` over the active sectors and the read of its counts
   std::vector<int64_t> slot_rounds(2, 0);  // pair-rounds enqueued per slot
+  // The work items of every sector are uploaded once: a sector that has
+  // converged keeps its items, whose work-groups return at once (skip
+  // pointer), and the rounds launched cover the active sectors' widest.
+  std::vector<kern::RoundWork> work;
+  int max_ell = 0;
+  for (int i : which) {
+    const auto& sc = secs[i];
+    max_ell = std::max(max_ell, sc.ell);
+    for (int pi = 0; pi < sc.nb / 2; ++pi) work.push_back({sc.y_off, sc.w_off, sc.ell, sc.nb, sc.flag_idx, pi});
+  }
+  const kern::RoundWork* d_work = work.empty() ? nullptr : ctx->upload_vec(work);
   auto enqueue = [&](int sweep) {
-    std::vector<kern::RoundWork> work;
-    int max_nb = 0, max_ell = 0;
+    int max_nb = 0;
+    int64_t nactive = 0;
     for (int i = 0; i < ns; ++i) {
       if (!active[i]) continue;
-      const auto& sc = secs[i];
-      max_nb = std::max(max_nb, sc.nb);
-      max_ell = std::max(max_ell, sc.ell);
-      for (int pi = 0; pi < sc.nb / 2; ++pi) work.push_back({sc.y_off, sc.w_off, sc.ell, sc.nb, sc.flag_idx, pi});
+      max_nb = std::max(max_nb, secs[i].nb);
+      nactive += secs[i].nb / 2;
     }
-    if (work.empty()) return false;
+    if (nactive == 0) return false;
     const int slot = sweep & 1;
-    slot_rounds[size_t(slot)] = int64_t(work.size()) * (max_nb - 1);
+    slot_rounds[size_t(slot)] = nactive * (max_nb - 1);
     const int* skip = sweep > 0 ? d_cnt[slot ^ 1] : nullptr;
     kern::fill_i32(ctx->stream(), d_cnt[slot], ns, 0);
-    auto* d_work = ctx->upload_vec(work);
     for (int r = 0; r < max_nb - 1; ++r) {
       KernelTimer kt(*ctx, "svd_round");
       kern::svd_round(ctx->stream(), ab, d_work, int(work.size()), r, d_cnt[slot], skip, tol, inner,
