@@ -536,7 +536,7 @@ def bench_serving(args):
     for rate in rates:
         trace = serving.make_trace(args.requests, rate=rate, seed=5, L_range=L_range, D_choices=D_choices)
         srv = serving.Server(n_slots=args.slots, hbm_budget=int(args.hbm_gib * (1 << 30)),
-                             max_sweeps=args.sweeps or 40, max_krylov=args.lanczos)
+                             max_sweeps=args.sweeps or 40, max_krylov=args.serve_krylov)
 
         def progress(r, x):
             log(f"rate {rate}: request {r.rid} L={r.L} D={r.D} "
@@ -554,7 +554,7 @@ def bench_serving(args):
            "config": {"workload": f"serving: {args.requests} XXZ chain requests per rate, Poisson arrivals (seed 5), "
                                   f"L ~ U{{{L_range[0]}..{L_range[1]}}}, D ~ U{set(D_choices)}, delta ~ U[0.5,1.5], "
                                   f"Sz=0, converged to |dE/E| <= 1e-10 within {args.sweeps or 40} sweeps",
-                      "slots": args.slots, "hbm_budget_gib": args.hbm_gib, "lanczos": args.lanczos},
+                      "slots": args.slots, "hbm_budget_gib": args.hbm_gib, "lanczos": args.serve_krylov},
            "p50_latency_s": head["p50_latency_s"], "p99_latency_s": head["p99_latency_s"], "per_rate": per_rate,
            "roofline": None, "cpu_baseline": None}
     print(json.dumps(out))
@@ -579,6 +579,8 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--rates", type=str, default="", help="serving: comma-separated rates to sweep")
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent solve slots")
+    ap.add_argument("--serve-krylov", type=int, default=4,
+                    help="serving: Krylov vectors of the k-step Lanczos update (0: the one-step update)")
     ap.add_argument("--hbm-gib", type=float, default=160.0, help="serving: HBM budget for admission")
     ap.add_argument("--full-trace", action="store_true",
                     help="serving: L up to 128 and D up to 512 instead of L <= 64, D <= 256")

@@ -37,9 +37,18 @@ void gemm_f64(hipStream_t s, double* base, const GemmTile* tiles, int ntiles, co
               const GemmPair* pairs, double alpha, double beta);
 // Tile variants of the FP64 grouped GEMM (same bits as gemm_f64; a tile's
 // tm, tn count in units of the variant's TM x TN).  kTile64 is gemm_f64.
-enum GemmVariant : int { kTile64 = 0, kTile64db = 1, kTile128x64 = 2, kTile64x128 = 3, kTile128 = 4, kGemmVariants = 5 };
-constexpr int kVariantTM[kGemmVariants] = {64, 64, 128, 64, 128};
-constexpr int kVariantTN[kGemmVariants] = {64, 64, 64, 128, 128};
+// kTile64k32 is the double-buffered 64 x 64 tile with 32-deep k steps.
+enum GemmVariant : int {
+  kTile64 = 0,
+  kTile64db = 1,
+  kTile128x64 = 2,
+  kTile64x128 = 3,
+  kTile128 = 4,
+  kTile64k32 = 5,
+  kGemmVariants = 6
+};
+constexpr int kVariantTM[kGemmVariants] = {64, 64, 128, 64, 128, 64};
+constexpr int kVariantTN[kGemmVariants] = {64, 64, 64, 128, 128, 64};
 void gemm_f64_variant(hipStream_t s, int variant, double* base, const GemmTile* tiles, int ntiles,
                       const GemmOut* outs, const GemmPair* pairs, double alpha, double beta);
 

@@ -247,26 +247,27 @@ __global__ __launch_bounds__(256) void k_gemm_f64(double* __restrict__ base, con
 
 // ---- tile variants of the FP64 grouped GEMM ---------------------------------
 // TM x TN output tile per work-group (4 waves in 2 x 2, each (TM/2) x (TN/2)
-// as (TM/32) x (TN/32) MFMA 16x16 tiles), k in steps of BK = 16 through a
-// double-buffered LDS: the global loads of step t+1 are in registers while
-// the MFMAs of step t run, then written to the other buffer, one barrier per
-// step.  Every output element accumulates over k in the same order as
-// k_gemm_f64 (16-wide steps, 4-wide MFMA chunks, pairs in list order), so all
-// variants give the same bits.
-template <int TM, int TN>
+// as (TM/32) x (TN/32) MFMA 16x16 tiles), k in steps of KD (16 or 32)
+// through a double-buffered LDS: the global loads of step t+1 are in
+// registers while the MFMAs of step t run, then written to the other buffer,
+// one barrier per step.  Every output element accumulates over k in the same
+// order as k_gemm_f64 (4-wide MFMA chunks from each pair's k = 0, pairs in
+// list order; the zero padding of a pair's last step adds exact zeros), so
+// all variants give the same bits.
+template <int TM, int TN, int KD>
 struct TileRegs {
-  double a[TM / 16], b[TN / 16];
+  double a[TM * KD / 256], b[TN * KD / 256];
 };
 
-template <int TM, int TN>
+template <int TM, int TN, int KD>
 __device__ __forceinline__ void fetch_t(const double* __restrict__ base, const GemmPair& pr, int M, int N, int m0,
-                                        int n0, int k0, int tid, TileRegs<TM, TN>& r) {
-  constexpr int EA = TM / 16, EB = TN / 16;  // elements per thread
+                                        int n0, int k0, int tid, TileRegs<TM, TN, KD>& r) {
+  constexpr int EA = TM * KD / 256, EB = TN * KD / 256;  // elements per thread
   const double* A = base + pr.a_off;
   const double* B = base + pr.b_off;
   const int K = pr.K;
-  if (pr.a_cs == 1) {  // k contiguous: 16/EA threads per row, EA consecutive k each
-    constexpr int TPR = 16 / EA;
+  if (pr.a_cs == 1) {  // k contiguous: KD/EA threads per row, EA consecutive k each
+    constexpr int TPR = KD / EA;
     const int m = tid / TPR, kq = (tid % TPR) * EA, gm = m0 + m;
 #pragma unroll
     for (int j = 0; j < EA; ++j) {
@@ -290,8 +291,8 @@ __device__ __forceinline__ void fetch_t(const double* __restrict__ base, const G
       const int gn = n0 + nq + j;
       r.b[j] = (gn < N && gk < K) ? B[int64_t(gk) * pr.b_rs + gn] : 0.0;
     }
-  } else {  // k contiguous (or general): 16/EB threads per column
-    constexpr int TPC = 16 / EB;
+  } else {  // k contiguous (or general): KD/EB threads per column
+    constexpr int TPC = KD / EB;
     const int n = tid / TPC, kq = (tid % TPC) * EB, gn = n0 + n;
 #pragma unroll
     for (int j = 0; j < EB; ++j) {
@@ -301,12 +302,12 @@ __device__ __forceinline__ void fetch_t(const double* __restrict__ base, const G
   }
 }
 
-template <int TM, int TN>
-__device__ __forceinline__ void stash_t(const GemmPair& pr, int tid, const TileRegs<TM, TN>& r,
+template <int TM, int TN, int KD>
+__device__ __forceinline__ void stash_t(const GemmPair& pr, int tid, const TileRegs<TM, TN, KD>& r,
                                         double (*As)[TM + 16], double (*Bs)[TN + 16]) {
-  constexpr int EA = TM / 16, EB = TN / 16;
+  constexpr int EA = TM * KD / 256, EB = TN * KD / 256;
   if (pr.a_cs == 1) {
-    constexpr int TPR = 16 / EA;
+    constexpr int TPR = KD / EA;
     const int m = tid / TPR, kq = (tid % TPR) * EA;
 #pragma unroll
     for (int j = 0; j < EA; ++j) As[kq + j][m] = r.a[j];
@@ -322,20 +323,20 @@ __device__ __forceinline__ void stash_t(const GemmPair& pr, int tid, const TileR
 #pragma unroll
     for (int j = 0; j < EB; ++j) Bs[k][nq + j] = r.b[j];
   } else {
-    constexpr int TPC = 16 / EB;
+    constexpr int TPC = KD / EB;
     const int n = tid / TPC, kq = (tid % TPC) * EB;
 #pragma unroll
     for (int j = 0; j < EB; ++j) Bs[kq + j][n] = r.b[j];
   }
 }
 
-template <int TM, int TN>
+template <int TM, int TN, int KD>
 __global__ __launch_bounds__(256) void k_gemm_f64t(double* __restrict__ base, const GemmTile* __restrict__ tiles,
                                                    const GemmOut* __restrict__ outs,
                                                    const GemmPair* __restrict__ pairs, double alpha, double beta) {
   constexpr int WM = TM / 2, WN = TN / 2, FI = WM / 16, FJ = WN / 16;
-  __shared__ double As[2][BK][TM + 16];
-  __shared__ double Bs[2][BK][TN + 16];
+  __shared__ double As[2][KD][TM + 16];
+  __shared__ double Bs[2][KD][TN + 16];
   const GemmTile t = tiles[blockIdx.x];
   const GemmOut o = outs[t.out];
   const int m0 = t.tm * TM, n0 = t.tn * TN;
@@ -352,13 +353,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64t(double* __restrict__ base, co
   if (p < o.pair_end) {
     GemmPair pr = pairs[p];
     int k0 = 0, buf = 0;
-    TileRegs<TM, TN> regs;
-    fetch_t<TM, TN>(base, pr, o.M, o.N, m0, n0, 0, tid, regs);
-    stash_t<TM, TN>(pr, tid, regs, As[0], Bs[0]);
+    TileRegs<TM, TN, KD> regs;
+    fetch_t<TM, TN, KD>(base, pr, o.M, o.N, m0, n0, 0, tid, regs);
+    stash_t<TM, TN, KD>(pr, tid, regs, As[0], Bs[0]);
     __syncthreads();
     // the step after (p, k0)
     auto advance = [&](int& q, int& k, GemmPair& g) {
-      k += BK;
+      k += KD;
       if (k >= g.K) {
         k = 0;
         ++q;
@@ -370,10 +371,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64t(double* __restrict__ base, co
     GemmPair npr = pr;
     advance(np, nk, npr);
     bool more = np < o.pair_end;
-    if (more) fetch_t<TM, TN>(base, npr, o.M, o.N, m0, n0, nk, tid, regs);
+    if (more) fetch_t<TM, TN, KD>(base, npr, o.M, o.N, m0, n0, nk, tid, regs);
     while (true) {
 #pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
+      for (int kk = 0; kk < KD; kk += 4) {
         const int kr = kk + (lane >> 4), c = lane & 15;
         double a[FI], b[FJ];
 #pragma unroll
@@ -388,14 +389,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64t(double* __restrict__ base, co
       if (!more) break;
       // regs hold step (np, nk): into the other buffer (its last readers
       // passed the barrier of the previous step), then fetch the step after
-      stash_t<TM, TN>(npr, tid, regs, As[buf ^ 1], Bs[buf ^ 1]);
+      stash_t<TM, TN, KD>(npr, tid, regs, As[buf ^ 1], Bs[buf ^ 1]);
       GemmPair cur = npr;
       int q = np, k = nk;
       advance(q, k, npr);
       more = q < o.pair_end;
       np = q;
       nk = k;
-      if (more) fetch_t<TM, TN>(base, npr, o.M, o.N, m0, n0, nk, tid, regs);
+      if (more) fetch_t<TM, TN, KD>(base, npr, o.M, o.N, m0, n0, nk, tid, regs);
       (void)cur;
       __syncthreads();
       buf ^= 1;
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64t(double* __restrict__ base, co
       // was the last one loaded
       if (!more) {
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
+        for (int kk = 0; kk < KD; kk += 4) {
           const int kr = kk + (lane >> 4), c = lane & 15;
           double a[FI], b[FJ];
 #pragma unroll
@@ -573,18 +574,22 @@ void gemm_f64_variant(hipStream_t s, int variant, double* base, const GemmTile* 
       hipLaunchKernelGGL(k_gemm_f64, dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
       break;
     case kTile64db:
-      hipLaunchKernelGGL((k_gemm_f64t<64, 64>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
+      hipLaunchKernelGGL((k_gemm_f64t<64, 64, 16>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha, beta);
       break;
     case kTile128x64:
-      hipLaunchKernelGGL((k_gemm_f64t<128, 64>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
+      hipLaunchKernelGGL((k_gemm_f64t<128, 64, 16>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
                          beta);
       break;
     case kTile64x128:
-      hipLaunchKernelGGL((k_gemm_f64t<64, 128>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
+      hipLaunchKernelGGL((k_gemm_f64t<64, 128, 16>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
                          beta);
       break;
     case kTile128:
-      hipLaunchKernelGGL((k_gemm_f64t<128, 128>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
+      hipLaunchKernelGGL((k_gemm_f64t<128, 128, 16>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
+                         beta);
+      break;
+    case kTile64k32:
+      hipLaunchKernelGGL((k_gemm_f64t<64, 64, 32>), dim3(ntiles), dim3(256), 0, s, base, tiles, outs, pairs, alpha,
                          beta);
       break;
     default:

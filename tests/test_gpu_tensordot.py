@@ -198,11 +198,11 @@ def _variant_cases(rng):
     return out
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_gemm_tile_variants_give_the_same_bits(ctx, variant, monkeypatch):
-    """Every FP64 tile variant (64x64 double-buffered, 128x64, 64x128, 128x128)
-    against the 64x64 kernel: bit-identical blocks (same per-element
-    accumulation order)."""
+    """Every FP64 tile variant (64x64 double-buffered, 128x64, 64x128, 128x128,
+    64x64 with 32-deep k steps) against the 64x64 kernel: bit-identical blocks
+    (same per-element accumulation order)."""
     rng = np.random.default_rng(40 + variant)
     for a, b, da, db in _variant_cases(rng):
         monkeypatch.setenv("QTT_GEMM_VARIANT", "0")

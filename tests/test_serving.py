@@ -24,7 +24,7 @@ def test_footprint_grows_with_size():
 
 def test_admission_respects_budget_and_slots():
     trace = [serving.Request(i, 0.001 * i, 32, 128, 1.0, i) for i in range(12)]
-    need = serving.footprint_bytes(32, 128)
+    need = serving.footprint_bytes(32, 128, krylov=4)   # the server's default k-step mode
     budget = int(2.5 * need)          # at most two fit at once
     live, peak, lock = [0], [0], threading.Lock()
 
@@ -49,7 +49,7 @@ def test_admission_respects_budget_and_slots():
 def test_oversize_request_is_rejected_not_blocking():
     big = serving.Request(0, 0.0, 128, 512, 1.0, 1)
     small = serving.Request(1, 0.0, 16, 64, 1.0, 2)
-    srv = serving.Server(n_slots=2, hbm_budget=serving.footprint_bytes(16, 64) * 2,
+    srv = serving.Server(n_slots=2, hbm_budget=serving.footprint_bytes(16, 64, krylov=4) * 2,
                          solve_fn=lambda ctx, req: (1.0, 1), context_fn=lambda n: object())
     res, _ = srv.run([big, small])
     assert isinstance(res[0], MemoryError) and isinstance(res[1], serving.Result)
