@@ -296,6 +296,7 @@ def headline(args):
         out["chain_L32_D128"] = sub_chain(qa, args)
         out["contract_svd_D1024"] = sub_contract_svd(qa, args)
         out["grouped_batch"] = sub_grouped(qa, args)
+        out["serving"] = sub_serving(args)
     print(json.dumps(out))
 
 
@@ -406,6 +407,22 @@ def sub_contract_svd(qa, args, steps=None):
                                                          f"CPU calls, median {t * 1e3:.1f} ms/step"}
     ctx.close()
     return res
+
+
+def sub_serving(args, n=8, rate=4.0):
+    """A bounded serving trace for the default line: n seeded Poisson XXZ
+    requests (L in 16..64, D in {64, 128, 256}) at `rate` per second on 4
+    slots in the served mode (4-vector Lanczos); the full trace and the rate
+    sweep run with --workload serving --full-trace --rates ..."""
+    from quantit_amd import serving
+    log("serving: bounded trace")
+    trace = serving.make_trace(n, rate=rate, seed=5, L_range=(16, 64), D_choices=(64, 128, 256))
+    srv = serving.Server(n_slots=4, hbm_budget=160 << 30, max_sweeps=40, max_krylov=args.serve_krylov)
+    res, wall = srv.run(trace)
+    summ = serving.summarize(res, wall)
+    summ.update({"requests": n, "rate": rate, "max_krylov": args.serve_krylov,
+                 "trace": "L ~ U{16..64} even, D ~ U{64,128,256}, delta ~ U[0.5,1.5], Sz=0, |dE/E| <= 1e-10"})
+    return summ
 
 
 def sub_grouped(qa, args):
