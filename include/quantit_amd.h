@@ -209,6 +209,22 @@ int qtt_dmrg(int L, qtt_btensor* const* mpo, qtt_btensor** mps, const qtt_dmrg_o
  * at oc instead of 0: sweeps start toward the far end, a centre on the last
  * site starts one site to the left, and the centre is moved back to oc at
  * the end (details::dmrg_impl, sources/dmrg.cpp:224-236, :257-266). */
+/* dmrg(bMPO&, bMPS&, options, dmrg_logger&) with the logger's hooks
+ * (include/dmrg_logger.h:27-108; called as sources/dmrg.cpp:239-268 does):
+ * init(user, options) before the first sweep; it_log_all(user, iteration,
+ * energy, L, state) after every sweep; end_log_all(user, iteration, energy,
+ * L, state) once the orthogonality centre is back at oc.  The state handles
+ * are lent for the duration of the callback (the callback may pass them to
+ * any entry point that does not destroy them, and must not keep them).
+ * Null function pointers are skipped.  Otherwise as qtt_dmrg_from_center. */
+typedef struct qtt_dmrg_logger {
+  void* user;
+  void (*init)(void* user, const qtt_dmrg_options* options);
+  void (*it_log_all)(void* user, size_t iteration, double energy, int L, qtt_btensor* const* state);
+  void (*end_log_all)(void* user, size_t iteration, double energy, int L, qtt_btensor* const* state);
+} qtt_dmrg_logger;
+int qtt_dmrg_logged(int L, qtt_btensor* const* mpo, qtt_btensor** mps, int oc, const qtt_dmrg_options* options,
+                    const qtt_dmrg_logger* logger, double* energy);
 int qtt_dmrg_from_center(int L, qtt_btensor* const* mpo, qtt_btensor** mps, int oc, const qtt_dmrg_options* options,
                          double* energy, int max_log, double* sweep_energies, double* sweep_ms, int64_t* max_bond,
                          int* n_sweeps, double* contract_flops, double* svd_flops);

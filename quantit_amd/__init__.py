@@ -72,6 +72,15 @@ class DmrgOptionsC(C.Structure):
                 ("minimum_bond", C.c_uint64), ("maximum_iterations", C.c_uint64)]
 
 
+LOGGER_INIT_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(DmrgOptionsC))
+LOGGER_LOG_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, C.c_double, C.c_int, C.POINTER(C.c_void_p))
+
+
+class DmrgLoggerC(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("init", LOGGER_INIT_FN), ("it_log_all", LOGGER_LOG_FN),
+                ("end_log_all", LOGGER_LOG_FN)]
+
+
 # every symbol of include/quantit_amd.h with its argument types
 SIGNATURES = {
     "qtt_last_error": (C.c_char_p, []),
@@ -119,6 +128,7 @@ SIGNATURES = {
     "qtt_hamil2site_times_state": (C.c_int, [P, P, P, P, PP]),
     "qtt_one_step_lanczos": (C.c_int, [P, P, P, P, PP, DP, DP, DP]),
     "qtt_eig2x2": (C.c_int, [P, C.c_double, C.c_double, C.c_double, DP, DP, DP]),
+    "qtt_dmrg_logged": (C.c_int, [C.c_int, PP, PP, C.c_int, C.POINTER(DmrgOptionsC), C.POINTER(DmrgLoggerC), DP]),
     "qtt_dmrg_from_center": (C.c_int, [C.c_int, PP, PP, C.c_int, C.POINTER(DmrgOptionsC), DP, C.c_int, DP, DP, I64P,
                                        C.POINTER(C.c_int), DP, DP]),
     "qtt_mps_move_oc": (C.c_int, [C.c_int, PP, C.c_int, C.c_int]),
@@ -465,7 +475,44 @@ class DmrgOptions:
         self.maximum_iterations = maximum_iterations
 
 
-def dmrg(mpo, mps, options, oc=0, seed=0):
+class DmrgLogger:
+    """``dmrg_logger`` (include/dmrg_logger.h:27-108): dmrg calls init(options)
+    before the first sweep, it_log_all(iteration, energy, state) after every
+    sweep and end_log_all(iteration, energy, state) at the end; by default
+    both go to log_all, which calls log_step, log_energy and log_bond_dims.
+    ``state`` is the list of the MPS tensors, lent for the call only (the
+    handles are invalidated when it returns)."""
+
+    def init(self, options):
+        pass
+
+    def log_step(self, iteration):
+        pass
+
+    def log_energy(self, energy):
+        pass
+
+    def log_bond_dims(self, state):
+        pass
+
+    def log_all(self, iteration, energy, state):
+        self.log_step(iteration)
+        self.log_energy(energy)
+        self.log_bond_dims(state)
+
+    def it_log_all(self, iteration, energy, state):
+        self.log_all(iteration, energy, state)
+
+    def end_log_all(self, iteration, energy, state):
+        self.log_all(iteration, energy, state)
+
+
+def bond_dims(state):
+    """Right bond dimension of every MPS tensor (sum of its last leg's sections)."""
+    return [int(sum(t.structure()[1][-1])) for t in state]
+
+
+def dmrg(mpo, mps, options, oc=0, seed=0, logger=None):
     """``dmrg(bMPO&, bMPS&, options, logger)`` (include/dmrg.h:39-40) on a
     state whose orthogonality centre is at site ``oc`` (moved back there at
     the end, as the reference does).  Returns (energy, new mps list, log
@@ -495,6 +542,8 @@ def dmrg(mpo, mps, options, oc=0, seed=0):
         t.h = None
     o = DmrgOptionsC(options.cutoff, options.convergence_criterion, min(options.maximum_bond, 2 ** 64 - 1),
                      options.minimum_bond, options.maximum_iterations)
+    if logger is not None:
+        return _dmrg_logged(mpo, mpo_h, mps_h, L, int(oc), o, options, logger)
     nmax = int(min(options.maximum_iterations, 100000))
     E = C.c_double()
     es = np.zeros(max(nmax, 1))
@@ -511,6 +560,51 @@ def dmrg(mpo, mps, options, oc=0, seed=0):
     n = ns.value
     log = {"energies": es[:n].tolist(), "sweep_ms": ms[:n].tolist(), "max_bond": mb[:n].tolist(),
            "contract_flops": cf.value, "svd_flops": sf.value}
+    return E.value, new_mps, log
+
+
+def _dmrg_logged(mpo, mpo_h, mps_h, L, oc, o, options, logger):
+    """qtt_dmrg_logged with a Python logger object (see DmrgLogger).  An
+    exception raised by a hook is kept and raised once dmrg returns."""
+    ctx = mpo[0].ctx
+    err = []
+    log = {"energies": [], "max_bond": []}
+
+    def lent(n, hs):
+        return [BTensor(ctx, C.c_void_p(hs[i])) for i in range(n)]
+
+    def call(fn, *a):
+        if err:
+            return
+        try:
+            fn(*a)
+        except BaseException as e:  # noqa: BLE001 - reported after dmrg returns
+            err.append(e)
+
+    def on_init(_user, _opts):
+        call(getattr(logger, "init", lambda _o: None), options)
+
+    def on_it(_user, it, E, n, hs):
+        state = lent(n, hs)
+        log["energies"].append(E)
+        log["max_bond"].append(max(bond_dims(state)))
+        call(getattr(logger, "it_log_all", lambda *_: None), int(it), float(E), state)
+        for t in state:  # the handles are lent: the library releases them
+            t.h = None
+
+    def on_end(_user, it, E, n, hs):
+        state = lent(n, hs)
+        call(getattr(logger, "end_log_all", lambda *_: None), int(it), float(E), state)
+        for t in state:
+            t.h = None
+
+    cb = DmrgLoggerC(None, LOGGER_INIT_FN(on_init), LOGGER_LOG_FN(on_it), LOGGER_LOG_FN(on_end))
+    E = C.c_double()
+    st = library().qtt_dmrg_logged(L, C.cast(mpo_h, PP), C.cast(mps_h, PP), oc, C.byref(o), C.byref(cb), C.byref(E))
+    new_mps = [BTensor(ctx, C.c_void_p(mps_h[i])) for i in range(L)]
+    _check(st)
+    if err:
+        raise err[0]
     return E.value, new_mps, log
 
 

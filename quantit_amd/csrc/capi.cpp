@@ -641,6 +641,51 @@ int qtt_dmrg_from_center(int L, qtt_btensor* const* mpo, qtt_btensor** mps, int 
   });
 }
 
+int qtt_dmrg_logged(int L, qtt_btensor* const* mpo, qtt_btensor** mps, int oc, const qtt_dmrg_options* options,
+                    const qtt_dmrg_logger* logger, double* energy) {
+  return guard([&] {
+    need(options, "options");
+    need(logger, "logger");
+    check_length(L);
+    qtt::DmrgState st;
+    st.oc = oc;
+    need(mps, "mps");
+    st.mps = gather(L, mps, "mps", nullptr);
+    st.mpo = gather(L, mpo, "mpo", st.mps[0].ctx);
+    for (auto& t : st.mps)
+      if (t.ctx != st.mps[0].ctx) throw qtt::InvalidArgument("tensors belong to different contexts");
+    qtt::DmrgOptions opt;
+    opt.cutoff = options->cutoff;
+    opt.convergence_criterion = options->convergence_criterion;
+    opt.maximum_bond = size_t(options->maximum_bond);
+    opt.minimum_bond = size_t(options->minimum_bond);
+    opt.maximum_iterations = size_t(options->maximum_iterations);
+    // the state is lent to the callback as handles valid for the call only
+    auto lend = [&](void (*fn)(void*, size_t, double, int, qtt_btensor* const*), size_t it, double E,
+                    const std::vector<qtt::BTensor>& state) {
+      std::vector<qtt_btensor*> hs;
+      hs.reserve(state.size());
+      for (const auto& t : state) hs.push_back(wrap(qtt::BTensor(t)));
+      fn(logger->user, it, E, int(hs.size()), hs.data());
+      for (auto* h : hs) release(h);
+    };
+    qtt::DmrgHooks hooks;
+    if (logger->init) hooks.init = [&] { logger->init(logger->user, options); };
+    if (logger->it_log_all)
+      hooks.it_log_all = [&](size_t it, double E, const std::vector<qtt::BTensor>& s) {
+        lend(logger->it_log_all, it, E, s);
+      };
+    if (logger->end_log_all)
+      hooks.end_log_all = [&](size_t it, double E, const std::vector<qtt::BTensor>& s) {
+        lend(logger->end_log_all, it, E, s);
+      };
+    qtt::DmrgLog log;
+    const double E = qtt::dmrg_run(st, opt, &log, &hooks);
+    replace(L, mps, st.mps);
+    if (energy) *energy = E;
+  });
+}
+
 int qtt_hamil2site_times_state(const qtt_btensor* theta, const qtt_btensor* W12, const qtt_btensor* L,
                                const qtt_btensor* R, qtt_btensor** out) {
   return guard([&] {

@@ -249,7 +249,7 @@ double dmrg_sweep(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
   return item(E);
 }
 
-double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
+double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log, const DmrgHooks* hooks) {
   dmrg_prepare(st);
   const int L = int(st.mpo.size());
   // details::dmrg_impl (dmrg.cpp:224-236): sweeps start toward the far end
@@ -262,10 +262,13 @@ double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
     --st.oc;
   }
   double E0 = 100000.0;
-  for (size_t it = 0; it < opt.maximum_iterations; ++it) {
+  if (hooks && hooks->init) hooks->init();
+  size_t it = 0;
+  for (; it < opt.maximum_iterations; ++it) {
     auto t0 = std::chrono::steady_clock::now();
     const double E = dmrg_sweep(st, opt, log);
     auto t1 = std::chrono::steady_clock::now();
+    if (hooks && hooks->it_log_all) hooks->it_log_all(it, E, st.mps);
     if (log) {
       log->energies.push_back(E);
       log->sweep_ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
@@ -285,6 +288,7 @@ double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log) {
                                      std::to_string(st.oc) + ". original oc: " + std::to_string(init_pos));
     mps_move_oc(st.mps, st.oc, init_pos);
   }
+  if (hooks && hooks->end_log_all) hooks->end_log_all(it, E0, st.mps);
   return E0;
 }
 

@@ -1,6 +1,7 @@
 // Two-site DMRG driver on device-resident tensors (sources/dmrg.cpp).
 #pragma once
 
+#include <functional>
 #include <limits>
 #include <vector>
 
@@ -45,9 +46,18 @@ void dmrg_prepare(DmrgState& st);
 BTensor dmrg_update(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
 // One full sweep: 2(L-2) updates (dmrg.cpp:127-142); returns the last energy.
 double dmrg_sweep(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
+// The calls dmrg_impl makes on its dmrg_logger (sources/dmrg.cpp:239-268):
+// init before the first sweep, it_log_all(iteration, E, state) after every
+// sweep, end_log_all(iteration, E, state) once the centre is back where it
+// started.  Empty functions are skipped.
+struct DmrgHooks {
+  std::function<void()> init;
+  std::function<void(size_t, double, const std::vector<BTensor>&)> it_log_all;
+  std::function<void(size_t, double, const std::vector<BTensor>&)> end_log_all;
+};
 // details::dmrg_impl (dmrg.cpp:219-273) from the orthogonality centre
 // st.oc; the centre is moved back there at the end.
-double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log);
+double dmrg_run(DmrgState& st, const DmrgOptions& opt, DmrgLog* log, const DmrgHooks* hooks = nullptr);
 
 // Environment helpers (dmrg.cpp:334-345, 424-493).
 Shape edge_shape_prep(const Shape& t, int dim);

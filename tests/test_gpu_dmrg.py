@@ -254,3 +254,54 @@ def test_dmrg_random_start_matches_oracle(ctx, q):
     for a, b in zip((from_dev(t) for t in new), mps_ref):
         assert a.secs == b.secs and a.cvals == b.cvals and a.sel == b.sel
         assert sorted(a.blocks) == sorted(b.blocks)
+
+
+def test_dmrg_logger_hooks(ctx):
+    """dmrg with a dmrg_logger (include/dmrg_logger.h:27-108, called as
+    sources/dmrg.cpp:239-268 does): init once with the options, it_log_all
+    after every sweep with the sweep's energy and the live state (usable
+    through the ABI during the call: <H> of that state equals the sweep's
+    energy up to the truncation), end_log_all once at the end; the energies
+    are the same bits as without a logger, and an exception raised by a hook
+    comes back after dmrg returns."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(6)
+    L = 10
+    dense = dor.xxz_field_dense(0.8, rng.uniform(-1.0, 1.0, L))
+    mpo, mps = _setup(dense, seed=8)
+    kw = dict(cutoff=1e-10, convergence_criterion=1e-9, maximum_bond=32, minimum_bond=4, maximum_iterations=8)
+    dmpo = [to_dev(ctx, t) for t in mpo]
+    E_plain, _, log_plain = qa.dmrg(dmpo, [to_dev(ctx, t) for t in mps], qa.DmrgOptions(**kw))
+
+    class Rec(qa.DmrgLogger):
+        def __init__(self):
+            self.calls = []
+
+        def init(self, options):
+            self.calls.append(("init", options.maximum_iterations))
+
+        def it_log_all(self, it, E, state):
+            nrm = qa.contract_norm(state, state)
+            h = qa.contract_energy(state, state, dmpo) / nrm
+            self.calls.append(("it", it, E, h, max(qa.bond_dims(state))))
+
+        def end_log_all(self, it, E, state):
+            self.calls.append(("end", it, E, len(state)))
+
+    rec = Rec()
+    E, new, log = qa.dmrg(dmpo, [to_dev(ctx, t) for t in mps], qa.DmrgOptions(**kw), logger=rec)
+    its = [c for c in rec.calls if c[0] == "it"]
+    assert rec.calls[0] == ("init", 8) and rec.calls[-1][0] == "end"
+    assert [c[1] for c in its] == list(range(len(its)))
+    assert [c[2] for c in its] == log_plain["energies"] and E == E_plain
+    for c in its:
+        assert abs(c[3] - c[2]) <= 1e-7 * abs(c[2]), c
+    assert rec.calls[-1] == ("end", len(its) - 1, E, L)
+    assert len(new) == L and abs(qa.contract_norm(new, new) - 1.0) < 1e-10
+
+    class Boom(qa.DmrgLogger):
+        def it_log_all(self, it, E, state):
+            raise KeyError("from the hook")
+
+    with pytest.raises(KeyError):
+        qa.dmrg(dmpo, [to_dev(ctx, t) for t in mps], qa.DmrgOptions(**kw), logger=Boom())
