@@ -420,6 +420,7 @@ def sub_serving(args, n=8, rate=4.0):
     srv = serving.Server(n_slots=4, hbm_budget=160 << 30, max_sweeps=40, max_krylov=args.serve_krylov)
     res, wall = srv.run(trace)
     summ = serving.summarize(res, wall)
+    summ["hbm_reserved_peak_gib"] = (srv.budget - srv.min_free) / float(1 << 30)
     summ.update({"requests": n, "rate": rate, "max_krylov": args.serve_krylov,
                  "trace": "L ~ U{16..64} even, D ~ U{64,128,256}, delta ~ U[0.5,1.5], Sz=0, |dE/E| <= 1e-10"})
     return summ
@@ -563,6 +564,9 @@ def bench_serving(args):
         res, wall = srv.run(trace, progress=progress)
         summ = serving.summarize(res, wall)
         summ["rate"] = rate
+        # peak of the admission model's reservations (footprint_bytes of the
+        # solves running at once): the HBM the served mix held at most
+        summ["hbm_reserved_peak_gib"] = (srv.budget - srv.min_free) / float(1 << 30)
         per_rate.append(summ)
     head = per_rate[-1]
     out = {"metric": METRIC, "value": head["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
