@@ -316,6 +316,22 @@ def sub_chain(qa, args):
     p = solve(qa, ctx, mpo_parts, mps_parts, opts, profile=True)
     res["roofline"] = kernel_roofline(p["kt"], p["log"]["contract_flops"], p["svd_flops"], p["svd_flops_wide"], None)
     res["kernel_ms_per_solve"] = {k: round(v[0], 3) for k, v in sorted(p["kt"].items(), key=lambda kv: -kv[1][0])[:8]}
+    # the same solve with the k-step Lanczos eigensolver at tolerance 1e-12
+    # (32 vectors, matvec replayed as a HIP graph), beside the reference's
+    # one-step update above
+    log("chain: k-step Lanczos, tol 1e-12")
+    ctx.set_lanczos(32, 1e-12)
+    solve(qa, ctx, mpo_parts, mps_parts, opts)
+    st0 = ctx.lanczos_stats()
+    lz = solve(qa, ctx, mpo_parts, mps_parts, opts)
+    st1 = ctx.lanczos_stats()
+    res["lanczos_tol_1e-12"] = {"max_krylov": 32, "sweep_ms": lz["log"]["sweep_ms"],
+                                "mean_s_per_sweep": float(np.mean(lz["log"]["sweep_ms"])) / 1e3,
+                                "energies": lz["log"]["energies"],
+                                "krylov_vectors_per_update": (st1["vectors"] - st0["vectors"]) /
+                                max(1, st1["updates"] - st0["updates"]),
+                                "graph_replays": st1["graph_replays"] - st0["graph_replays"]}
+    ctx.set_lanczos(0, 1e-12)
     if not args.no_cpu_baseline:
         log("chain: cpu baseline")
         from oracle import torch_cpu as tc
