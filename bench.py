@@ -331,6 +331,11 @@ def sub_chain(qa, args):
                                 "krylov_vectors_per_update": (st1["vectors"] - st0["vectors"]) /
                                 max(1, st1["updates"] - st0["updates"]),
                                 "graph_replays": st1["graph_replays"] - st0["graph_replays"]}
+    ctx.set_graphs(False)   # the same solve with every matvec launched eagerly
+    eg = solve(qa, ctx, mpo_parts, mps_parts, opts)
+    ctx.set_graphs(True)
+    res["lanczos_tol_1e-12"]["eager_mean_s_per_sweep"] = float(np.mean(eg["log"]["sweep_ms"])) / 1e3
+    res["lanczos_tol_1e-12"]["eager_energies_equal"] = eg["log"]["energies"] == lz["log"]["energies"]
     ctx.set_lanczos(0, 1e-12)
     if not args.no_cpu_baseline:
         log("chain: cpu baseline")
