@@ -432,17 +432,17 @@ def sub_contract_svd(qa, args, steps=None):
 
 def sub_serving(args, n=8, rate=4.0):
     """A bounded serving trace for the default line: n seeded Poisson XXZ
-    requests (L in 16..64, D in {64, 128, 256}) at `rate` per second on 4
-    slots in the served mode (4-vector Lanczos); the full trace and the rate
+    requests (L in 16..64, D in {64, 128, 256}) at `rate` per second on
+    --slots slots in the served mode (4-vector Lanczos); the full trace and the rate
     sweep run with --workload serving --full-trace --rates ..."""
     from quantit_amd import serving
     log("serving: bounded trace")
     trace = serving.make_trace(n, rate=rate, seed=5, L_range=(16, 64), D_choices=(64, 128, 256))
-    srv = serving.Server(n_slots=4, hbm_budget=160 << 30, max_sweeps=40, max_krylov=args.serve_krylov)
+    srv = serving.Server(n_slots=args.slots, hbm_budget=160 << 30, max_sweeps=40, max_krylov=args.serve_krylov)
     res, wall = srv.run(trace)
     summ = serving.summarize(res, wall)
     summ["hbm_reserved_peak_gib"] = (srv.budget - srv.min_free) / float(1 << 30)
-    summ.update({"requests": n, "rate": rate, "max_krylov": args.serve_krylov,
+    summ.update({"requests": n, "rate": rate, "slots": args.slots, "max_krylov": args.serve_krylov,
                  "trace": "L ~ U{16..64} even, D ~ U{64,128,256}, delta ~ U[0.5,1.5], Sz=0, |dE/E| <= 1e-10"})
     return summ
 
@@ -620,7 +620,7 @@ def main():
     ap.add_argument("--requests", type=int, default=8, help="serving: trace length per rate")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--rates", type=str, default="", help="serving: comma-separated rates to sweep")
-    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent solve slots")
+    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent solve slots")
     ap.add_argument("--serve-krylov", type=int, default=4,
                     help="serving: Krylov vectors of the k-step Lanczos update (0: the one-step update)")
     ap.add_argument("--hbm-gib", type=float, default=160.0, help="serving: HBM budget for admission")

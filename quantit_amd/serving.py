@@ -109,7 +109,7 @@ def solve(ctx, req, cutoff=1e-10, criterion=1e-10, max_sweeps=40):
 class Server:
     """n_slots solve slots sharing an HBM budget (bytes)."""
 
-    def __init__(self, n_slots=4, hbm_budget=200 << 30, device=0, solve_fn=None, context_fn=None, max_krylov=4,
+    def __init__(self, n_slots=8, hbm_budget=200 << 30, device=0, solve_fn=None, context_fn=None, max_krylov=4,
                  **solve_kw):
         """max_krylov: Krylov vectors of the k-step Lanczos update (2..64;
         4, the default, served the full trace fastest with every request
