@@ -192,10 +192,19 @@ struct RoundWork {
 };
 void svd_round(hipStream_t s, double* base, const RoundWork* work, int nwork, int round, int* counters,
                const int* skip, double tol, int inner_sweeps, int max_ell, int cross_after0);
-// norms + ranks (one work-group per sector), tiled transposed scatter of the
-// (sector, part, row0, col0) tasks into U / V, orthonormal completion.
+// norms + ranks (one work-group per sector) and the tiled transposed scatter
+// of the (sector, part, row0, col0) tasks into U / V.
 void svd_finalize(hipStream_t s, double* base, const SvdSector* secs, const SvdSector* host_secs, int nsec,
                   const int4* tasks, int ntasks);
+// Orthonormal completion of the Z-side columns (Z: ell x k row-major, values
+// S descending) whose singular value is numerically zero, for columns
+// t < limit only: a truncation that keeps `limit` columns never reads the
+// others.  Scratch: v (ell doubles) and p (k doubles).
+struct CompleteTask {
+  int64_t z_off, s_off, v_off, p_off;
+  int32_t ell, k, limit, pad;
+};
+void svd_complete(hipStream_t s, double* base, const CompleteTask* tasks, int n);
 // eigh mode: c = ||A||_F per sector, stored at shift_off, and Y += c I.
 void svd_shift(hipStream_t s, double* base, const SvdSector* secs, int nsec);
 void fill_i32(hipStream_t s, int32_t* p, int n, int32_t v);

@@ -1422,49 +1422,64 @@ __global__ __launch_bounds__(256) void k_svd_write(double* __restrict__ base, co
 }
 
 // Orthonormal completion of the Z-side columns whose singular value is
-// numerically zero (modified Gram-Schmidt, twice, against all earlier
-// columns, starting from unit vectors).  One work-group per sector.
+// numerically zero, columns t < limit: starting from unit vectors, classical
+// Gram-Schmidt twice against all earlier columns (the dot products of one
+// pass in one step, thread per column; the update thread per row), keeping
+// the first candidate whose residual norm exceeds 1/2.  One work-group per
+// sector.
 __global__ __launch_bounds__(256) void k_svd_complete(double* __restrict__ base,
-                                                      const SvdSector* __restrict__ secs) {
+                                                      const CompleteTask* __restrict__ ts) {
   __shared__ double sh[256];
-  const SvdSector s = secs[blockIdx.x];
-  if (s.eig_shift) return;
-  const int k = s.k, ell = s.ell, kp = s.nb * kSvdBlock;
-  const double* S = base + s.s_off;
-  double* v = base + s.w_off + int64_t(kp) * kp + 2 * kp;
-  double* Zout = base + (s.transposed ? s.v_off : s.u_off);  // ell x k row-major
-  const double smax = k > 0 ? S[0] : 0.0;
-  const double null_thr = smax * double(ell) * DBL_EPSILON;
+  const CompleteTask T = ts[blockIdx.x];
+  const int k = T.k, ell = T.ell, lim = T.limit < k ? T.limit : k;
+  if (lim <= 0) return;
+  const double* S = base + T.s_off;
+  double* v = base + T.v_off;
+  double* p = base + T.p_off;
+  double* Z = base + T.z_off;  // ell x k row-major
+  const double null_thr = S[0] * double(ell) * DBL_EPSILON;
   const int tid = threadIdx.x;
-  int first_null = k;
-  for (int t = 0; t < k; ++t)
+  int first_null = lim;
+  for (int t = 0; t < lim; ++t)
     if (!(S[t] > null_thr && S[t] > 0.0)) {
       first_null = t;
       break;
     }
-  for (int t = first_null; t < k; ++t) {
+  for (int t = first_null; t < lim; ++t) {
     for (int cand = 0; cand < ell; ++cand) {
       const int e0 = (t + cand) % ell;
       for (int l = tid; l < ell; l += blockDim.x) v[l] = (l == e0) ? 1.0 : 0.0;
       __syncthreads();
-      for (int pass = 0; pass < 2; ++pass)
-        for (int c = 0; c < t; ++c) {
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int c = tid; c < t; c += blockDim.x) {
           double acc = 0.0;
-          for (int l = tid; l < ell; l += blockDim.x) acc += Zout[int64_t(l) * k + c] * v[l];
-          const double pr = wg_sum(acc, sh);
-          for (int l = tid; l < ell; l += blockDim.x) v[l] -= pr * Zout[int64_t(l) * k + c];
-          __syncthreads();
+          for (int l = 0; l < ell; ++l) acc = fma(Z[int64_t(l) * k + c], v[l], acc);
+          p[c] = acc;
         }
+        __syncthreads();
+        for (int l = tid; l < ell; l += blockDim.x) {
+          const double* zr = Z + int64_t(l) * k;
+          double acc = 0.0;
+          for (int c = 0; c < t; ++c) acc = fma(zr[c], p[c], acc);
+          v[l] -= acc;
+        }
+        __syncthreads();
+      }
       double acc = 0.0;
       for (int l = tid; l < ell; l += blockDim.x) acc += v[l] * v[l];
       const double nrm = sqrt(wg_sum(acc, sh));
       if (nrm > 0.5) {
-        for (int l = tid; l < ell; l += blockDim.x) Zout[int64_t(l) * k + t] = v[l] / nrm;
+        for (int l = tid; l < ell; l += blockDim.x) Z[int64_t(l) * k + t] = v[l] / nrm;
         __syncthreads();
         break;
       }
     }
   }
+}
+
+void svd_complete(hipStream_t st, double* base, const CompleteTask* tasks, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_svd_complete, dim3(n), dim3(256), 0, st, base, tasks);
 }
 
 void svd_finalize(hipStream_t st, double* base, const SvdSector* secs, const SvdSector* host_secs, int nsec,
@@ -1475,7 +1490,6 @@ void svd_finalize(hipStream_t st, double* base, const SvdSector* secs, const Svd
   hipLaunchKernelGGL(k_svd_colnorms, dim3(nsec, (kmax + 15) / 16), dim3(256), 0, st, base, secs);
   hipLaunchKernelGGL(k_svd_norms, dim3(nsec), dim3(256), 0, st, base, secs);
   if (ntasks > 0) hipLaunchKernelGGL(k_svd_write, dim3(ntasks), dim3(256), 0, st, base, secs, tasks);
-  hipLaunchKernelGGL(k_svd_complete, dim3(nsec), dim3(256), 0, st, base, secs);
 }
 
 __global__ __launch_bounds__(256) void k_svd_shift(double* __restrict__ base, const SvdSector* __restrict__ secs) {

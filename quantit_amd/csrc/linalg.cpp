@@ -4,6 +4,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <functional>
 #include <limits>
 #include <numeric>
 
@@ -82,6 +83,9 @@ struct Batched {
   int sweeps = 0;
   int unconverged = 0;  // sectors still rotating when a sweep cap was reached
   std::vector<char> wide;  // sectors whose rotations ran as multi-work-group rounds (k_svd_round)
+  // defer_complete: the orthonormal completion of null columns, one task per
+  // sector (scratch offsets unset), left to the caller
+  std::vector<kern::CompleteTask> complete;
 };
 
 // A grouped GEMM whose descriptors are on the device (one staging copy), so
@@ -227,7 +231,8 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
 // Jacobi on G rotates nothing in its first sweep, i.e. the columns of Y are
 // orthogonal to the tolerance.  Every step is an exact orthogonal transform
 // of Y, so the singular values still come from the column norms of Y.
-Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool eigh_mode) {
+Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool eigh_mode,
+                    bool defer_complete = false) {
   Context* ctx = t.ctx;
   double* ab = arena_base(ctx);
   Batched B;
@@ -747,6 +752,26 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
     kern::svd_finalize(ctx->stream(), ab, ctx->upload_vec(B.secs), B.secs.data(), ns, ctx->upload_vec(wt),
                        int(wt.size()));
   }
+  if (!eigh_mode) {
+    for (const auto& s : B.secs) {
+      const int64_t kp = int64_t(s.nb) * kern::kSvdBlock;
+      kern::CompleteTask c{};
+      c.z_off = s.transposed ? s.v_off : s.u_off;
+      c.s_off = s.s_off;
+      c.ell = s.ell;
+      c.k = s.k;
+      c.limit = s.k;
+      // scratch in the sector's work area: W has been written out by now
+      c.p_off = s.w_off;
+      c.v_off = s.w_off + kp * kp + 2 * kp;
+      B.complete.push_back(c);
+    }
+    if (!defer_complete) {
+      KernelTimer kt(*ctx, "svd_complete");
+      kern::svd_complete(ctx->stream(), ab, ctx->upload_vec(B.complete), int(B.complete.size()));
+      B.complete.clear();
+    }
+  }
   // the work buffers are released after the finalize kernels in stream order
   return B;
 }
@@ -755,14 +780,18 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
 
 // svd(const btensor&, some, uv) on the rank-2 reshaped tensor + reshape_as
 // back (sources/btensor_linalg.cpp:390-534).
-SvdResult svd(const BTensor& t, size_t split, SvdStats* stats) {
+// deferred (may be null): receives the completion tasks of the null columns
+// instead of running them (one per sector = section of d).
+static SvdResult svd_impl(const BTensor& t, size_t split, SvdStats* stats,
+                          std::vector<kern::CompleteTask>* deferred) {
   Context* ctx = t.ctx;
   if (split > size_t(t.rank())) throw InvalidArgument("split larger than the rank");
   BTensor rt = reshape(t, {int64_t(split)});
   const Shape& rs = rt.shape;
   const Groups& g = rs.g;
   auto plans = compact_dense_plan(rt);
-  Batched B = run_batched(rt, plans, false);
+  Batched B = run_batched(rt, plans, false, deferred != nullptr);
+  if (deferred) *deferred = B.complete;
   const int ns = int(plans.size());
   // d (neutral), U (rows, k) with the row charges and rule = rule of t,
   // V (cols, k) with inverse charges and neutral rule
@@ -850,6 +879,8 @@ SvdResult svd(const BTensor& t, size_t split, SvdStats* stats) {
   return out;
 }
 
+SvdResult svd(const BTensor& t, size_t split, SvdStats* stats) { return svd_impl(t, split, stats, nullptr); }
+
 // eigh(const btensor&, split) (sources/btensor_linalg.cpp:294-389): shifted
 // symmetric Jacobi, eigenvalues ascending per sector.
 EighResult eigh(const BTensor& t, size_t split) {
@@ -934,8 +965,12 @@ namespace {
 // sections (U and V of an svd, the eigenvectors of an eigh).  The unitaries
 // keep their block order, sliced blocks are compacted into fresh storage;
 // *units are replaced and the truncated d is returned.
+// before_copy (may be empty) runs once the kept counts are known, before any
+// block is copied, with the kept count of every section of d (0 for sections
+// without a block).
 BTensor truncate_units(BTensor&& d_in, const std::vector<BTensor*>& units, size_t max_size, size_t min_size,
-                       double tol, double pw) {
+                       double tol, double pw,
+                       const std::function<void(const std::vector<int32_t>&)>& before_copy = {}) {
   BTensor d = std::move(d_in);
   Context* ctx = d.ctx;
   double* ab = arena_base(ctx);
@@ -965,6 +1000,11 @@ BTensor truncate_units(BTensor&& d_in, const std::vector<BTensor*>& units, size_
   }
   std::vector<int32_t> kept(nd);
   ctx->read_small(kept.data(), d_kept, size_t(nd) * sizeof(int32_t));
+  if (before_copy) {
+    std::vector<int32_t> kept_sec(d.shape.secs[0].size(), 0);
+    for (int i = 0; i < nd; ++i) kept_sec[d.blocks[i].idx.i[0]] = kept[i];
+    before_copy(kept_sec);
+  }
 
   const size_t nu = units.size();
   std::vector<std::vector<Block>> ub(nu);
@@ -1108,9 +1148,39 @@ SvdResult svd_matrix(const BTensor& t, bool some, bool compute_uv) {
   return r;
 }
 
+// The truncation drops every column whose singular value is numerically zero
+// unless min_size keeps it, so the orthonormal completion of such columns runs
+// after the cut and only for the columns that are kept.
 SvdResult svd_truncated(const BTensor& t, size_t split, double tol, size_t min_size, size_t max_size, double pw,
                         SvdStats* stats) {
-  return truncate(svd(t, split, stats), max_size, min_size, tol, pw);
+  std::vector<kern::CompleteTask> pend;
+  SvdResult out = svd_impl(t, split, stats, &pend);
+  Context* ctx = t.ctx;
+  auto complete_kept = [&](const std::vector<int32_t>& kept_sec) {
+    std::vector<kern::CompleteTask> run;
+    int64_t elems = 0;
+    for (size_t i = 0; i < pend.size() && i < kept_sec.size(); ++i) {
+      if (kept_sec[i] <= 0 || pend[i].k <= 0) continue;
+      kern::CompleteTask c = pend[i];
+      c.limit = kept_sec[i];
+      c.v_off = elems;  // relative to the scratch buffer until it exists
+      elems += c.ell;
+      c.p_off = elems;
+      elems += c.k;
+      run.push_back(c);
+    }
+    if (run.empty()) return;
+    BufPtr scratch = ctx->alloc(size_t(elems) * sizeof(double));
+    const int64_t so = scratch->dptr() - arena_base(ctx);
+    for (auto& c : run) {
+      c.v_off += so;
+      c.p_off += so;
+    }
+    KernelTimer kt(*ctx, "svd_complete");
+    kern::svd_complete(ctx->stream(), arena_base(ctx), ctx->upload_vec(run), int(run.size()));
+  };
+  out.d = truncate_units(std::move(out.d), {&out.U, &out.V}, max_size, min_size, tol, pw, complete_kept);
+  return out;
 }
 
 }  // namespace qtt

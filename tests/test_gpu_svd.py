@@ -167,6 +167,40 @@ def test_svd_rank_deficient_orthonormal(ctx):
         assert np.abs(M.T @ M - np.eye(M.shape[1])).max() < 1e-12
 
 
+@pytest.mark.parametrize("keep_all", [False, True])
+def test_truncated_svd_completes_kept_null_columns(ctx, keep_all):
+    """The truncating svd completes null columns only after the cut and only
+    those it keeps: its U, d, V equal svd followed by truncate bit for bit,
+    both when the null columns are dropped (min_size 1) and when min_size
+    keeps every column; kept U and V are orthonormal and U d V^T = A."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(37)
+    rows = rand_leg(rng, 4, smax=24)
+    A = rand_btensor(rng, G1, [rows, inv_leg(G1, rows)], (0,))
+    for k in A.blocks:
+        b = A.blocks[k]
+        if b.shape[0] > 2:
+            b[-1] = b[0]          # two null directions per sector
+            b[-2] = 2.0 * b[1]
+    total = sum(min(b.shape) for b in A.blocks.values())
+    mins = total if keep_all else 1
+    t = to_dev(ctx, A)
+    fused = [from_dev(x) for x in qa.svd(t, 1, 1e-10, mins, 2 ** 40, 2.0)]
+    U0, d0, V0 = qa.svd(t, 1)
+    split = [from_dev(x) for x in qa.truncate(U0, d0, V0, 2 ** 40, mins, 1e-10, 2.0)]
+    for a, b in zip(fused, split):
+        assert_same_structure(a, b)
+        for k in a.blocks:
+            assert np.array_equal(a.blocks[k], b.blocks[k])
+    U, d, V = fused
+    kept = sum(b.size for b in d.blocks.values())
+    assert kept == total if keep_all else kept < total
+    _check_svd(A, U, d, V, 1)
+    for x in (U, V):
+        M = x.to_dense()
+        assert np.abs(M.T @ M - np.eye(M.shape[1])).max() < 1e-12
+
+
 def test_eigh_matches_dense(ctx):
     import quantit_amd as qa
     rng = np.random.default_rng(41)
