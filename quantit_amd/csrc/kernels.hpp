@@ -257,9 +257,10 @@ void gram_vectors(hipStream_t s, double* base, const void* descs, int n, int kma
 
 // Global truncation (sources/btensor_linalg.cpp:657-755,
 // sources/LinearAlgebra.cpp:57-75): singular values of nsec sectors stored
-// descending at s_off[i] with k[i] entries; writes kept[i].
-void truncate_select(hipStream_t s, double* base, const int64_t* s_offs, const int32_t* ks, int nsec,
-                     int64_t scratch_off, int64_t total, double tol, double pow, int64_t min_size,
+// descending at s_off[i] with k[i] entries, s_pos[i] = k[0] + ... + k[i-1];
+// writes kept[i].
+void truncate_select(hipStream_t s, double* base, const int64_t* s_offs, const int32_t* ks, const int64_t* s_pos,
+                     int nsec, int64_t scratch_off, int64_t total, double tol, double pow, int64_t min_size,
                      int64_t max_size, int32_t* kept, double* thr_out);
 
 }  // namespace kern

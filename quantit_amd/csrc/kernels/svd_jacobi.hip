@@ -1340,8 +1340,11 @@ __global__ __launch_bounds__(256) void k_svd_colnorms(double* __restrict__ base,
 }
 
 // One work-group per sector: rank of every singular value in descending order
-// (ties broken by column index) and the sorted values.
+// (ties broken by column index) and the sorted values.  The values are
+// compared from LDS, 1024 at a time.
 __global__ __launch_bounds__(256) void k_svd_norms(double* __restrict__ base, const SvdSector* __restrict__ secs) {
+  constexpr int kChunk = 1024;
+  __shared__ double chunk[kChunk];
   const SvdSector s = secs[blockIdx.x];
   const int k = s.k, kp = s.nb * kSvdBlock;
   double* aux = base + s.w_off + int64_t(kp) * kp;
@@ -1349,18 +1352,27 @@ __global__ __launch_bounds__(256) void k_svd_norms(double* __restrict__ base, co
   double* rnk = aux + kp;
   double* S = base + s.s_off;
   const int tid = threadIdx.x;
-  for (int j = tid; j < k; j += blockDim.x) {
-    const double sj = sig[j];
+  for (int j0 = 0; j0 < k; j0 += blockDim.x) {  // uniform trip count: every thread meets the barriers
+    const int j = j0 + tid;
+    const double sj = j < k ? sig[j] : 0.0;
     int r = 0;
-    for (int i = 0; i < k; ++i) {
-      const double si = sig[i];
-      r += (si > sj) || (si == sj && i < j);
+    for (int i0 = 0; i0 < k; i0 += kChunk) {
+      const int n = min(kChunk, k - i0);
+      __syncthreads();  // the previous chunk has been read
+      for (int t = tid; t < n; t += blockDim.x) chunk[t] = sig[i0 + t];
+      __syncthreads();
+      for (int t = 0; t < n; ++t) {
+        const double si = chunk[t];
+        r += (si > sj) || (si == sj && i0 + t < j);
+      }
     }
-    rnk[j] = double(r);
-    if (s.eig_shift)
-      S[k - 1 - r] = sj - base[s.shift_off];
-    else
-      S[r] = sj;
+    if (j < k) {
+      rnk[j] = double(r);
+      if (s.eig_shift)
+        S[k - 1 - r] = sj - base[s.shift_off];
+      else
+        S[r] = sj;
+    }
   }
 }
 
@@ -1525,7 +1537,8 @@ void fill_i32(hipStream_t st, int32_t* p, int n, int32_t v) {
 // is found by the reference's sequential tail accumulation.
 template <bool IN_LDS>
 __global__ __launch_bounds__(1024) void k_truncate(double* __restrict__ base, const int64_t* __restrict__ s_offs,
-                                                   const int32_t* __restrict__ ks, int nsec, int64_t scratch_off,
+                                                   const int32_t* __restrict__ ks, const int64_t* __restrict__ s_pos,
+                                                   int nsec, int64_t scratch_off,
                                                    int64_t total, double tol, double pw, int64_t min_size,
                                                    int64_t max_size, int32_t* __restrict__ kept,
                                                    double* __restrict__ thr_out) {
@@ -1535,16 +1548,16 @@ __global__ __launch_bounds__(1024) void k_truncate(double* __restrict__ base, co
   int64_t n2 = 1;
   while (n2 < total) n2 <<= 1;
   const int tid = threadIdx.x;
-  // gather (sector lists concatenated in order), pad with -inf
-  {
-    int64_t pos = 0;
-    for (int i = 0; i < nsec; ++i) {
-      const double* src = base + s_offs[i];
-      for (int j = tid; j < ks[i]; j += blockDim.x) v[pos + j] = src[j];
-      pos += ks[i];
-    }
-    for (int64_t j = total + tid; j < n2; j += blockDim.x) v[j] = -INFINITY;
+  const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  // gather (sector lists concatenated in order, s_pos[i] = start of sector i),
+  // one wave per sector, pad with -inf
+  for (int i = wave; i < nsec; i += nw) {
+    const double* src = base + s_offs[i];
+    const int64_t pos = s_pos[i];
+    const int n = ks[i];
+    for (int j = lane; j < n; j += 64) v[pos + j] = src[j];
   }
+  for (int64_t j = total + tid; j < n2; j += blockDim.x) v[j] = -INFINITY;
   __syncthreads();
   // bitonic sort, descending
   for (int64_t size = 2; size <= n2; size <<= 1) {
@@ -1593,26 +1606,25 @@ __global__ __launch_bounds__(1024) void k_truncate(double* __restrict__ base, co
   __syncthreads();
   const double thr = thr_sh;
   // kept[i] = index of the first value of sector i that is not above thr
-  // (its length when none), one wave per sector, 64 values per ballot
-  const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  // (its length when none), one wave per sector, 64 values per ballot; the
+  // chunks are all read (no early exit) so that their loads overlap
   for (int i = wave; i < nsec; i += nw) {
     const double* src = base + s_offs[i];
-    int c = ks[i];
-    for (int j0 = 0; j0 < ks[i]; j0 += 64) {
+    const int n = ks[i];
+    int c = n;
+#pragma unroll 4
+    for (int j0 = 0; j0 < n; j0 += 64) {
       const int j = j0 + lane;
-      const unsigned long long m = __ballot(j < ks[i] && !(src[j < ks[i] ? j : 0] > thr));
-      if (m) {
-        c = j0 + __ffsll((long long)m) - 1;
-        break;
-      }
+      const unsigned long long m = __ballot(j < n && !(src[j < n ? j : 0] > thr));
+      if (m && c == n) c = j0 + __ffsll((long long)m) - 1;
     }
     if (lane == 0) kept[i] = c;
   }
 }
 
-void truncate_select(hipStream_t st, double* base, const int64_t* s_offs, const int32_t* ks, int nsec,
-                     int64_t scratch_off, int64_t total, double tol, double pw, int64_t min_size, int64_t max_size,
-                     int32_t* kept, double* thr_out) {
+void truncate_select(hipStream_t st, double* base, const int64_t* s_offs, const int32_t* ks, const int64_t* s_pos,
+                     int nsec, int64_t scratch_off, int64_t total, double tol, double pw, int64_t min_size,
+                     int64_t max_size, int32_t* kept, double* thr_out) {
   int64_t n2 = 1;
   while (n2 < total) n2 <<= 1;
   const size_t lds = size_t(n2) * sizeof(double);
@@ -1625,11 +1637,11 @@ void truncate_select(hipStream_t st, double* base, const int64_t* s_offs, const 
     return size_t(v) - 256;
   }();
   if (lds <= limit)
-    hipLaunchKernelGGL(k_truncate<true>, dim3(1), dim3(1024), lds, st, base, s_offs, ks, nsec, scratch_off, total,
-                       tol, pw, min_size, max_size, kept, thr_out);
+    hipLaunchKernelGGL(k_truncate<true>, dim3(1), dim3(1024), lds, st, base, s_offs, ks, s_pos, nsec, scratch_off,
+                       total, tol, pw, min_size, max_size, kept, thr_out);
   else
-    hipLaunchKernelGGL(k_truncate<false>, dim3(1), dim3(1024), 0, st, base, s_offs, ks, nsec, scratch_off, total,
-                       tol, pw, min_size, max_size, kept, thr_out);
+    hipLaunchKernelGGL(k_truncate<false>, dim3(1), dim3(1024), 0, st, base, s_offs, ks, s_pos, nsec, scratch_off,
+                       total, tol, pw, min_size, max_size, kept, thr_out);
 }
 
 }  // namespace kern

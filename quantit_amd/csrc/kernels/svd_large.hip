@@ -14,6 +14,7 @@
 //   k_gram_vectors  normalises the columns of the rotated L into W and
 //                   completes zero columns from the unit vector with the
 //                   largest residual (two projection passes).
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <stdexcept>
@@ -62,10 +63,19 @@ __device__ __forceinline__ double fast_rsqrt_pc(double x) {
 
 constexpr int kLargeThreads = 1024;
 
-// flags[flag_idx] = pairs failing both tests, flags[aux_idx] = pairs passing
-// only the absolute one (same tests as k_gram_eig, on squares).  The
-// diagonal sits in LDS (dynamic, k doubles); wave w scans the upper part of
-// columns w, w + 16, ... with coalesced reads.
+__global__ void k_gram_check_zero(const SmallJacobi* __restrict__ js, int n, int* __restrict__ flags) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const SmallJacobi J = js[i];
+    if (J.flag_idx >= 0) flags[J.flag_idx] = 0;
+    if (J.aux_idx >= 0) flags[J.aux_idx] = 0;
+  }
+}
+
+// flags[flag_idx] += pairs failing both tests, flags[aux_idx] += pairs passing
+// only the absolute one (same tests as k_gram_eig, on squares), both zeroed
+// beforehand.  Work-group y of a sector takes the columns c = y (mod
+// gridDim.y); the diagonal sits in LDS (dynamic, k doubles); wave w scans the
+// upper part of its w-th, (w + 16)-th, ... column with coalesced reads.
 __global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __restrict__ base,
                                                               const SmallJacobi* __restrict__ js,
                                                               int* __restrict__ flags, double tol) {
@@ -76,6 +86,7 @@ __global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __re
   const int k = J.k;
   const double* G = base + J.y_off;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const int ny = gridDim.y, y = blockIdx.y;
   if (tid == 0) nv = na = 0;
   double dm = 0.0;
   for (int i = tid; i < k; i += blockDim.x) {
@@ -85,7 +96,7 @@ __global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __re
   const double g2max = block_max(dm, sh);  // also orders the dg writes before the reads
   const double t2 = tol * tol;
   int my = 0, mya = 0;
-  for (int c = wave; c < k; c += nwave) {
+  for (int c = y + ny * wave; c < k; c += ny * nwave) {
     const double gjj = dg[c];
     const double* col = G + int64_t(c) * J.ldy;
 #pragma unroll 4
@@ -104,8 +115,8 @@ __global__ __launch_bounds__(kLargeThreads) void k_gram_check(const double* __re
   if (mya) atomicAdd(&na, mya);
   __syncthreads();
   if (tid == 0) {
-    if (J.flag_idx >= 0) flags[J.flag_idx] = nv;
-    if (J.aux_idx >= 0) flags[J.aux_idx] = na;
+    if (J.flag_idx >= 0 && nv) atomicAdd(&flags[J.flag_idx], nv);
+    if (J.aux_idx >= 0 && na) atomicAdd(&flags[J.aux_idx], na);
   }
 }
 
@@ -122,7 +133,9 @@ void gram_check(hipStream_t st, const double* base, const void* descs, int n, in
   }();
   const size_t lds = size_t(kmax) * sizeof(double);
   if (lds > limit) throw std::runtime_error("gram check: sector wider than the LDS holds");
-  hipLaunchKernelGGL(k_gram_check, dim3(n), dim3(kLargeThreads), lds, st, base,
+  const int ny = std::max(1, std::min(8, kmax / 64));  // about 64 columns per work-group
+  hipLaunchKernelGGL(k_gram_check_zero, dim3(1), dim3(256), 0, st, static_cast<const SmallJacobi*>(descs), n, flags);
+  hipLaunchKernelGGL(k_gram_check, dim3(n, ny), dim3(kLargeThreads), lds, st, base,
                      static_cast<const SmallJacobi*>(descs), flags, tol);
 }
 

@@ -976,12 +976,13 @@ BTensor truncate_units(BTensor&& d_in, const std::vector<BTensor*>& units, size_
   double* ab = arena_base(ctx);
   const int nd = int(d.blocks.size());
   if (nd == 0) return d;
-  std::vector<int64_t> s_offs(nd);
+  std::vector<int64_t> s_offs(nd), s_pos(nd);
   std::vector<int32_t> ks(nd);
   int64_t total = 0;
   for (int i = 0; i < nd; ++i) {
     s_offs[i] = d.ptr(d.blocks[i]) - ab;
     ks[i] = int32_t(d.blocks[i].numel);
+    s_pos[i] = total;
     total += ks[i];
   }
   if (total == 0) return d;
@@ -993,8 +994,8 @@ BTensor truncate_units(BTensor&& d_in, const std::vector<BTensor*>& units, size_
   const int64_t clamp = std::numeric_limits<int64_t>::max();
   {
     KernelTimer kt(*ctx, "truncate_select");
-    auto [up_s_offs, up_ks] = ctx->upload_pack(s_offs, ks);
-    kern::truncate_select(ctx->stream(), ab, up_s_offs, up_ks, nd,
+    auto [up_s_offs, up_ks, up_pos] = ctx->upload_pack(s_offs, ks, s_pos);
+    kern::truncate_select(ctx->stream(), ab, up_s_offs, up_ks, up_pos, nd,
                           scratch->dptr() - ab, total, tol, pw, int64_t(std::min<size_t>(min_size, size_t(clamp))),
                           int64_t(std::min<size_t>(max_size, size_t(clamp))), d_kept, d_thr);
   }
