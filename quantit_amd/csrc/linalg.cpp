@@ -145,12 +145,22 @@ int env_int(const char* name, int dflt) {
 // as with a read between sweeps.  Once a sweep rotates under a quarter of its
 // pair-rounds the end is near and the next sweep is enqueued only after the
 // read (a speculative sweep after the last would be launches of no-ops).
+//
+// The sweep that ends the loop rotates nothing: it only confirms that every
+// column pair passes the round test.  Once a sweep rotates clearly fewer
+// pair-rounds than the sweeps before it (the tail of the quadratic
+// convergence), the sweep after it is followed by that test on all pairs at
+// once instead of a speculative sweep: G = Y^T Y for the active sectors (one
+// grouped GEMM) and the round's relative test on every pair of G
+// (k_gram_check).  A sector whose pairs all pass has converged: the sweep
+// after would have rotated nothing.
 int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& secs, const std::vector<int>& which,
                   double tol, int inner, int max_sweeps, std::vector<int>* first_rot, int* capped) {
   const int ns = int(secs.size());
   if (which.empty() || max_sweeps <= 0) return 0;
-  BufPtr cnt = ctx->alloc(size_t(2 * ns) * sizeof(int32_t));
+  BufPtr cnt = ctx->alloc(size_t(4 * ns) * sizeof(int32_t));  // counts of two sweeps, check flags
   int32_t* d_cnt[2] = {reinterpret_cast<int32_t*>(cnt->cptr()), reinterpret_cast<int32_t*>(cnt->cptr()) + ns};
+  int32_t* d_chk = reinterpret_cast<int32_t*>(cnt->cptr()) + 2 * ns;
   ctx->reserve_read(size_t(ns));
   std::vector<char> active(ns, 0);
   for (int i : which) active[i] = 1;
@@ -196,11 +206,56 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     QTT_HIP(hipEventRecord(ctx->read_event(slot), ctx->stream()));
     return true;
   };
+  // the all-pairs test after the sweep just enqueued (flags: failing pairs,
+  // relative and absolute, of sector i at d_chk[2 i], d_chk[2 i + 1])
+  BufPtr gram;
+  auto enqueue_check = [&] {
+    std::vector<kern::GemmOut> outs;
+    std::vector<kern::GemmPair> prs;
+    std::vector<kern::SmallJacobi> cd;
+    int64_t elems = 0;
+    int kc = 0;
+    for (int i = 0; i < ns; ++i)
+      if (active[i]) elems += int64_t(secs[i].k) * secs[i].k;
+    if (elems == 0) return false;
+    gram = ctx->alloc(size_t(elems) * sizeof(double));
+    int64_t go = gram->dptr() - ab;
+    for (int i = 0; i < ns; ++i) {
+      if (!active[i]) continue;
+      const auto& sc = secs[i];
+      kern::GemmPair p{};  // G = Y^T Y over the k columns (Y: ell x k, column-major)
+      p.a_off = sc.y_off;
+      p.a_rs = sc.ell;
+      p.a_cs = 1;
+      p.b_off = sc.y_off;
+      p.b_rs = 1;
+      p.b_cs = sc.ell;
+      p.K = sc.ell;
+      outs.push_back({go, sc.k, sc.k, int32_t(prs.size()), int32_t(prs.size() + 1), sc.k, 0});
+      prs.push_back(p);
+      cd.push_back({go, -1, sc.k, sc.k, sc.k, sc.k, 2 * i, 2 * i + 1});
+      kc = std::max(kc, sc.k);
+      go += int64_t(sc.k) * sc.k;
+    }
+    launch_gemm(ctx, ab, outs, prs, "svd_sweep_check_gram");
+    KernelTimer kt(*ctx, "svd_sweep_check");
+    kern::gram_check(ctx->stream(), ab, ctx->upload_vec(cd), int(cd.size()), d_chk, tol, kc);
+    return true;
+  };
+  const bool use_check = env_int("QTT_SVD_SWEEP_CHECK", 1) != 0;
   if (!enqueue(0)) return 0;
   int queued = 1, sweeps = 0;
-  bool speculate = true;
+  bool speculate = true, check_next = false;
+  double frac_max = 0.0;
+  std::vector<int32_t> h_chk(size_t(2 * ns)), h_zero;
   for (int sweep = 0; sweep < queued; ++sweep) {
-    if (speculate && queued < max_sweeps && enqueue(queued)) ++queued;
+    bool checked = false;
+    if (queued == sweep + 1) {
+      if (speculate && queued < max_sweeps && enqueue(queued))
+        ++queued;
+      else if (check_next && enqueue_check())
+        checked = true;
+    }
     ctx->wait_event(ctx->read_event(sweep & 1));
     const int32_t* h_cnt = ctx->read_slot(sweep & 1);
     sweeps = sweep + 1;
@@ -213,8 +268,26 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
       if (active[i] && h_cnt[i] == 0) active[i] = 0;
       any |= bool(active[i]);
     }
+    if (any && checked) {
+      ctx->read_small(h_chk.data(), d_chk, size_t(2 * ns) * sizeof(int32_t));
+      h_zero.assign(h_cnt, h_cnt + ns);
+      bool passed = false;
+      any = false;
+      for (int i = 0; i < ns; ++i) {
+        if (active[i] && h_chk[size_t(2 * i)] + h_chk[size_t(2 * i + 1)] == 0) {
+          active[i] = 0;
+          h_zero[size_t(i)] = 0;  // the next sweep's work-groups of this sector return at once
+          passed = true;
+        }
+        any |= bool(active[i]);
+      }
+      if (any && passed) ctx->stage_to(d_cnt[sweep & 1], h_zero.data(), size_t(ns) * sizeof(int32_t));
+    }
     if (!any) break;
-    speculate = 4 * rotated >= slot_rounds[size_t(sweep & 1)];
+    const double frac = double(rotated) / double(std::max<int64_t>(slot_rounds[size_t(sweep & 1)], 1));
+    frac_max = std::max(frac_max, frac);
+    speculate = 4 * rotated >= slot_rounds[size_t(sweep & 1)] && (!use_check || frac >= 0.75 * frac_max);
+    check_next = use_check && !speculate;
     if (queued == sweep + 1 && queued < max_sweeps && enqueue(queued)) ++queued;
   }
   if (capped && sweeps >= max_sweeps)
