@@ -433,7 +433,7 @@ def sub_contract_svd(qa, args, steps=None):
 def sub_serving(args, n=8, rate=4.0):
     """A bounded serving trace for the default line: n seeded Poisson XXZ
     requests (L in 16..64, D in {64, 128, 256}) at `rate` per second on
-    --slots slots in the served mode (4-vector Lanczos); the full trace and the rate
+    --slots slots in the served mode (k-step Lanczos, --serve-krylov); the full trace and the rate
     sweep run with --workload serving --full-trace --rates ..."""
     from quantit_amd import serving
     log("serving: bounded trace")
@@ -621,8 +621,9 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0, help="serving: Poisson arrival rate (1/s)")
     ap.add_argument("--rates", type=str, default="", help="serving: comma-separated rates to sweep")
     ap.add_argument("--slots", type=int, default=8, help="serving: concurrent solve slots")
-    ap.add_argument("--serve-krylov", type=int, default=4,
-                    help="serving: Krylov vectors of the k-step Lanczos update (0: the one-step update)")
+    ap.add_argument("--serve-krylov", type=lambda v: v if v == "auto" else int(v), default="auto",
+                    help="serving: Krylov vectors of the k-step Lanczos update (0: the one-step update; "
+                         "auto: 4 for D <= 256, 6 above)")
     ap.add_argument("--hbm-gib", type=float, default=160.0, help="serving: HBM budget for admission")
     ap.add_argument("--full-trace", action="store_true",
                     help="serving: L up to 128 and D up to 512 instead of L <= 64, D <= 256")

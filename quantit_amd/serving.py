@@ -109,12 +109,13 @@ def solve(ctx, req, cutoff=1e-10, criterion=1e-10, max_sweeps=40):
 class Server:
     """n_slots solve slots sharing an HBM budget (bytes)."""
 
-    def __init__(self, n_slots=8, hbm_budget=200 << 30, device=0, solve_fn=None, context_fn=None, max_krylov=4,
+    def __init__(self, n_slots=8, hbm_budget=200 << 30, device=0, solve_fn=None, context_fn=None, max_krylov="auto",
                  **solve_kw):
-        """max_krylov: Krylov vectors of the k-step Lanczos update (2..64;
-        4, the default, served the full trace fastest with every request
-        converged); 0 runs the reference's one-step update, which needs many
-        more sweeps."""
+        """max_krylov: Krylov vectors of the k-step Lanczos update (2..64),
+        0 for the reference's one-step update (which needs many more sweeps),
+        or "auto": 4 for requests with D <= 256 and 6 above, the sizes that
+        served the measured traces fastest (a sweep at D = 512 is costly
+        enough that fewer sweeps with more vectors per update win)."""
         self.n_slots = n_slots
         self.max_krylov = max_krylov
         self.budget = hbm_budget
@@ -131,9 +132,13 @@ class Server:
         # several solves at once: one stream each (a second stream per solve
         # would share hardware queues with the other solves' streams)
         ctx.set_overlap(self.n_slots == 1)
-        if self.max_krylov:
-            ctx.set_lanczos(self.max_krylov, 1e-12)
         return ctx
+
+    def krylov_for(self, r):
+        """Krylov vectors used for request r (0: one-step update)."""
+        if self.max_krylov == "auto":
+            return 4 if r.D <= 256 else 6
+        return int(self.max_krylov)
 
     def run(self, trace, time_scale=1.0, progress=None):
         """Replays the trace (arrival times scaled by time_scale) and returns
@@ -156,7 +161,7 @@ class Server:
                     state["next"] += 1
                 with self.cv:
                     for r in list(waiting):
-                        need = footprint_bytes(r.L, r.D, krylov=self.max_krylov)
+                        need = footprint_bytes(r.L, r.D, krylov=self.krylov_for(r))
                         if need > self.budget:
                             waiting.remove(r)
                             results[r.rid] = MemoryError(f"request {r.rid} needs {need} bytes")
@@ -181,6 +186,9 @@ class Server:
                 ctx = None
                 try:
                     ctx = self.context_fn(need)
+                    kr = self.krylov_for(r)
+                    if kr and hasattr(ctx, "set_lanczos"):
+                        ctx.set_lanczos(kr, 1e-12)
                     got = self.solve_fn(ctx, r, **self.solve_kw)
                     E, sweeps = got[0], got[1]
                     conv = bool(got[2]) if len(got) > 2 else True

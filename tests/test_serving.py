@@ -22,6 +22,17 @@ def test_footprint_grows_with_size():
     assert serving.footprint_bytes(32, 128) < serving.footprint_bytes(64, 128) < serving.footprint_bytes(64, 256)
 
 
+def test_krylov_size_per_request():
+    """auto: 4 vectors up to D = 256, 6 above; an explicit size applies to
+    every request; the footprint counts the request's own Krylov basis."""
+    small, large = serving.Request(0, 0.0, 32, 256, 1.0, 0), serving.Request(1, 0.0, 32, 512, 1.0, 1)
+    auto = serving.Server(n_slots=1, hbm_budget=1 << 40)
+    assert auto.krylov_for(small) == 4 and auto.krylov_for(large) == 6
+    fixed = serving.Server(n_slots=1, hbm_budget=1 << 40, max_krylov=8)
+    assert fixed.krylov_for(small) == fixed.krylov_for(large) == 8
+    assert serving.footprint_bytes(32, 512, krylov=6) > serving.footprint_bytes(32, 512, krylov=4)
+
+
 def test_admission_respects_budget_and_slots():
     trace = [serving.Request(i, 0.001 * i, 32, 128, 1.0, i) for i in range(12)]
     need = serving.footprint_bytes(32, 128, krylov=4)   # the server's default k-step mode
