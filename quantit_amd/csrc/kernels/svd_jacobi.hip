@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <stdexcept>
 
 #include "../kernels.hpp"
 
@@ -1339,40 +1340,43 @@ __global__ __launch_bounds__(256) void k_svd_colnorms(double* __restrict__ base,
   }
 }
 
-// One work-group per sector: rank of every singular value in descending order
-// (ties broken by column index) and the sorted values.  The values are
-// compared from LDS, 1024 at a time.
-__global__ __launch_bounds__(256) void k_svd_norms(double* __restrict__ base, const SvdSector* __restrict__ secs) {
+// Rank of every singular value of a sector in descending order (ties broken
+// by column index) and the sorted values.  Work-group (sector, y) is one wave
+// that ranks values 64 y ... 64 y + 63 against all k, compared from LDS 1024
+// at a time.
+constexpr int kNormsWave = 64;
+__global__ __launch_bounds__(kNormsWave) void k_svd_norms(double* __restrict__ base,
+                                                           const SvdSector* __restrict__ secs) {
   constexpr int kChunk = 1024;
   __shared__ double chunk[kChunk];
   const SvdSector s = secs[blockIdx.x];
   const int k = s.k, kp = s.nb * kSvdBlock;
+  const int j0 = blockIdx.y * kNormsWave;
+  if (j0 >= k) return;  // uniform
   double* aux = base + s.w_off + int64_t(kp) * kp;
   double* sig = aux;
   double* rnk = aux + kp;
   double* S = base + s.s_off;
   const int tid = threadIdx.x;
-  for (int j0 = 0; j0 < k; j0 += blockDim.x) {  // uniform trip count: every thread meets the barriers
-    const int j = j0 + tid;
-    const double sj = j < k ? sig[j] : 0.0;
-    int r = 0;
-    for (int i0 = 0; i0 < k; i0 += kChunk) {
-      const int n = min(kChunk, k - i0);
-      __syncthreads();  // the previous chunk has been read
-      for (int t = tid; t < n; t += blockDim.x) chunk[t] = sig[i0 + t];
-      __syncthreads();
-      for (int t = 0; t < n; ++t) {
-        const double si = chunk[t];
-        r += (si > sj) || (si == sj && i0 + t < j);
-      }
+  const int j = j0 + tid;
+  const double sj = j < k ? sig[j] : 0.0;
+  int r = 0;
+  for (int i0 = 0; i0 < k; i0 += kChunk) {
+    const int n = min(kChunk, k - i0);
+    __syncthreads();  // the previous chunk has been read
+    for (int t = tid; t < n; t += blockDim.x) chunk[t] = sig[i0 + t];
+    __syncthreads();
+    for (int t = 0; t < n; ++t) {
+      const double si = chunk[t];
+      r += (si > sj) || (si == sj && i0 + t < j);
     }
-    if (j < k) {
-      rnk[j] = double(r);
-      if (s.eig_shift)
-        S[k - 1 - r] = sj - base[s.shift_off];
-      else
-        S[r] = sj;
-    }
+  }
+  if (j < k) {
+    rnk[j] = double(r);
+    if (s.eig_shift)
+      S[k - 1 - r] = sj - base[s.shift_off];
+    else
+      S[r] = sj;
   }
 }
 
@@ -1500,7 +1504,7 @@ void svd_finalize(hipStream_t st, double* base, const SvdSector* secs, const Svd
   int kmax = 1;
   for (int i = 0; i < nsec; ++i) kmax = std::max(kmax, host_secs[i].k);
   hipLaunchKernelGGL(k_svd_colnorms, dim3(nsec, (kmax + 15) / 16), dim3(256), 0, st, base, secs);
-  hipLaunchKernelGGL(k_svd_norms, dim3(nsec), dim3(256), 0, st, base, secs);
+  hipLaunchKernelGGL(k_svd_norms, dim3(nsec, (kmax + kNormsWave - 1) / kNormsWave), dim3(kNormsWave), 0, st, base, secs);
   if (ntasks > 0) hipLaunchKernelGGL(k_svd_write, dim3(ntasks), dim3(256), 0, st, base, secs, tasks);
 }
 
@@ -1559,12 +1563,14 @@ __global__ __launch_bounds__(1024) void k_truncate(double* __restrict__ base, co
   }
   for (int64_t j = total + tid; j < n2; j += blockDim.x) v[j] = -INFINITY;
   __syncthreads();
-  // bitonic sort, descending
-  for (int64_t size = 2; size <= n2; size <<= 1) {
-    for (int64_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int64_t i = tid; i < n2 / 2; i += blockDim.x) {
-        const int64_t lo = 2 * stride * (i / stride) + (i % stride);
-        const int64_t hi = lo + stride;
+  // bitonic sort, descending (n2 < 2^31; strides are powers of two, so the
+  // pair index is formed with masks, not divisions)
+  const int m2 = int(n2);
+  for (int size = 2; size <= m2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < m2 / 2; i += blockDim.x) {
+        const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));
+        const int hi = lo + stride;
         const bool desc = ((lo & size) == 0);
         const double a = v[lo], b = v[hi];
         if (desc ? (a < b) : (a > b)) {
@@ -1627,6 +1633,7 @@ void truncate_select(hipStream_t st, double* base, const int64_t* s_offs, const 
                      int64_t max_size, int32_t* kept, double* thr_out) {
   int64_t n2 = 1;
   while (n2 < total) n2 <<= 1;
+  if (n2 > (int64_t(1) << 30)) throw std::runtime_error("truncation: more than 2^30 values");
   const size_t lds = size_t(n2) * sizeof(double);
   static const size_t limit = [] {
     int dev = 0, v = 0;
