@@ -1583,28 +1583,32 @@ __global__ __launch_bounds__(1024) void k_truncate(double* __restrict__ base, co
   }
   // compute_last_index (sources/LinearAlgebra.cpp:57-75), sequential as in the reference
   if (tid == 0) {
-    const bool sq = (pw == 2.0);
-    const double toln = sq ? tol * tol : pow(tol, pw);
-    int64_t last = total - 1;
-    double x = fabs(v[last]);
-    double trunc = sq ? x * x : pow(x, pw);
     // the same steps in the same order, the next eight values read ahead
-    // (a dependent read per step would dominate the scan)
-    for (bool done = false; !done;) {
-      double buf[8];
+    // (a dependent read per step would dominate the scan); the square and
+    // the general power are separate loops, so that the common pw = 2 scan
+    // evaluates no pow()
+    auto scan = [&](auto power) {
+      const double toln = power(tol);
+      int64_t last = total - 1;
+      double trunc = power(fabs(v[last]));
+      for (bool done = false; !done;) {
+        double buf[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) buf[j] = last - 1 - j >= 0 ? v[last - 1 - j] : 0.0;
+        for (int j = 0; j < 8; ++j) buf[j] = last - 1 - j >= 0 ? v[last - 1 - j] : 0.0;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if (!(last >= min_size && last > 0) || (trunc > toln && last < max_size)) {
-          done = true;
-          break;
+        for (int j = 0; j < 8; ++j) {
+          if (!(last >= min_size && last > 0) || (trunc > toln && last < max_size)) {
+            done = true;
+            break;
+          }
+          --last;
+          trunc += power(fabs(buf[j]));
         }
-        --last;
-        x = fabs(buf[j]);
-        trunc += sq ? x * x : pow(x, pw);
       }
-    }
+      return last;
+    };
+    const int64_t last = pw == 2.0 ? scan([](double x) { return x * x; })
+                                   : scan([pw](double x) { return pow(x, pw); });
     const double sl = v[last];
     thr_sh = sl - 2.0 * sl * DBL_EPSILON;
     thr_out[0] = thr_sh;
