@@ -1446,6 +1446,7 @@ __global__ __launch_bounds__(256) void k_svd_write(double* __restrict__ base, co
 __global__ __launch_bounds__(256) void k_svd_complete(double* __restrict__ base,
                                                       const CompleteTask* __restrict__ ts) {
   __shared__ double sh[256];
+  __shared__ int first_sh;
   const CompleteTask T = ts[blockIdx.x];
   const int k = T.k, ell = T.ell, lim = T.limit < k ? T.limit : k;
   if (lim <= 0) return;
@@ -1455,12 +1456,13 @@ __global__ __launch_bounds__(256) void k_svd_complete(double* __restrict__ base,
   double* Z = base + T.z_off;  // ell x k row-major
   const double null_thr = S[0] * double(ell) * DBL_EPSILON;
   const int tid = threadIdx.x;
-  int first_null = lim;
-  for (int t = 0; t < lim; ++t)
-    if (!(S[t] > null_thr && S[t] > 0.0)) {
-      first_null = t;
-      break;
-    }
+  // the first numerically zero value below the limit (lim when none)
+  if (tid == 0) first_sh = lim;
+  __syncthreads();
+  for (int t = tid; t < lim; t += blockDim.x)
+    if (!(S[t] > null_thr && S[t] > 0.0)) atomicMin(&first_sh, t);
+  __syncthreads();
+  const int first_null = first_sh;
   for (int t = first_null; t < lim; ++t) {
     for (int cand = 0; cand < ell; ++cand) {
       const int e0 = (t + cand) % ell;
