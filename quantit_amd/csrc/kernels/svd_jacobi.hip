@@ -81,6 +81,16 @@ __device__ __forceinline__ void jacobi_cs(double d, double h, double& c, double&
   sn = c * t;
 }
 
+// lane i of each 16-lane row <- lane (i + 1) mod 16 (DPP row rotate right by
+// 15), on a double as two 32-bit moves.
+__device__ __forceinline__ double dpp_row_next(double v) {
+  const long long b = __double_as_longlong(v);
+  int lo = int(b & 0xffffffffLL), hi = int(b >> 32);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x12F, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x12F, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
 // Apply the 32x32 rotation Q (LDS, pitch GP) to the 32 selected columns of a
 // column-major matrix M (rows x ld) in place, YC rows at a time (each wave
 // owns two 16-row MFMA tiles of a chunk).
@@ -333,6 +343,74 @@ __global__ __launch_bounds__(RESIDENT ? 1024 : 256) void k_svd_round(double* __r
         y = max(x0, x1);
       }
     };
+    if (cross && inner_sweeps == 1) {
+      // The same cross sweep with G and Q held in registers.  Thread (P, R)
+      // keeps g00 = G[P][R], g01 = G[P][b2], g10 = G[a2][R], g11 = G[a2][b2]
+      // and Q's (P, R), (P, b2), (P+16, R), (P+16, b2), where a2 = 16 +
+      // (P + r) mod 16 and b2 = 16 + (R + r) mod 16 move with the step r:
+      // the diagonal owner (P == R) holds pair P's 2x2 block and computes
+      // its rotation; after the update the moving columns come from lane
+      // R + 1 of the same 16-lane row (DPP row rotate) and the moving rows
+      // from thread P + 1 through LDS (G's area, which is dead from here
+      // on).  After 16 steps every value is back at its first position.
+      double (*s10)[NB2 / 2 + 1] = reinterpret_cast<double (*)[NB2 / 2 + 1]>(&G[0][0][0]);
+      double (*s11)[NB2 / 2 + 1] = s10 + NB2 / 2;
+      double g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0, qa0 = 0.0, qa1 = 0.0, qb0 = 0.0, qb1 = 0.0;
+      constexpr int H = NB2 / 2;
+      if (blk) {
+        g00 = G[0][P][R];
+        g01 = G[0][P][H + R];
+        g10 = G[0][H + P][R];
+        g11 = G[0][H + P][H + R];
+        qa0 = Q[0][P][R];
+        qa1 = Q[0][P][H + R];
+        qb0 = Q[0][H + P][R];
+        qb1 = Q[0][H + P][H + R];
+      }
+      __syncthreads();  // every read of G precedes the exchange writes into its area
+      for (int r = 0; r < H; ++r) {
+        if (blk && P == R) {
+          const double app = g00, aqq = g11, apq = g01;
+          const bool rot = app > 0.0 && aqq > 0.0 && apq * apq > (DBL_EPSILON * app) * (DBL_EPSILON * aqq);
+          double cc, ss;
+          jacobi_cs(aqq - app, 2.0 * apq, cc, ss);
+          rc[P] = rot ? cc : 1.0;
+          rs[P] = rot ? ss : 0.0;
+        }
+        __syncthreads();  // rotations published
+        if (blk) {
+          const double cp = rc[P], sp = rs[P], cq = rc[R], sq = rs[R];
+          const double h00 = cp * g00 - sp * g10, h01 = cp * g01 - sp * g11;
+          const double h10 = cp * g10 + sp * g00, h11 = cp * g11 + sp * g01;
+          g00 = cq * h00 - sq * h01;
+          g01 = cq * h01 + sq * h00;
+          g10 = cq * h10 - sq * h11;
+          g11 = cq * h11 + sq * h10;
+          const double na1 = cq * qa1 + sq * qa0, nb1 = cq * qb1 + sq * qb0;
+          qa0 = cq * qa0 - sq * qa1;
+          qb0 = cq * qb0 - sq * qb1;
+          g01 = dpp_row_next(g01);
+          g11 = dpp_row_next(g11);
+          qa1 = dpp_row_next(na1);
+          qb1 = dpp_row_next(nb1);
+          s10[P][R] = g10;
+          s11[P][R] = g11;
+        }
+        __syncthreads();  // moving rows published; the next step's rotations wait for these reads
+        if (blk) {
+          g10 = s10[(P + 1) & (H - 1)][R];
+          g11 = s11[(P + 1) & (H - 1)][R];
+        }
+      }
+      __syncthreads();
+      if (blk) {
+        Q[0][P][R] = qa0;
+        Q[0][P][H + R] = qa1;
+        Q[0][H + P][R] = qb0;
+        Q[0][H + P][H + R] = qb1;
+      }
+      __syncthreads();
+    } else
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
       if (tid == 0) nrot = 0;
       __syncthreads();
