@@ -244,6 +244,33 @@ def test_svd_large_graded_sector(ctx, m, n, decades):
     assert np.abs(Vd.T @ Vd - np.eye(k)).max() < 1e-12
 
 
+def test_sweep_tail_check_equals_confirming_sweep(ctx, monkeypatch):
+    """The all-pairs Gram test that ends the Jacobi sweeps in their tail
+    (QTT_SVD_SWEEP_CHECK=1, the default) and the confirming sweep it replaces
+    (=0) give the same singular values within 1e-12 of the largest, the same
+    structure, and both reconstruct A, on two wide graded sectors."""
+    import quantit_amd as qa
+    rng = np.random.default_rng(91)
+    A = BTensor(G1, [[300, 260], [280, 240]], [[(0,), (2,)], [(0,), (-2,)]], (0,))
+    for i, (m, n) in enumerate(((300, 280), (260, 240))):
+        k = min(m, n)
+        U0, _ = np.linalg.qr(rng.standard_normal((m, k)))
+        V0, _ = np.linalg.qr(rng.standard_normal((n, k)))
+        A.blocks[(i, i)] = (U0 * 10.0 ** (-9 * np.arange(k) / k)) @ V0.T
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("QTT_SVD_SWEEP_CHECK", flag)
+        U, d, V = (from_dev(x) for x in qa.svd(to_dev(ctx, A), 1))
+        _check_svd(A, U, d, V, 1)
+        out[flag] = (U, d, V)
+    (U1, d1, V1), (U0_, d0, V0_) = out["1"], out["0"]
+    assert_same_structure(d1, d0)
+    assert_same_structure(U1, U0_)
+    smax = max(float(b.max()) for b in d0.blocks.values())
+    for key in d0.blocks:
+        assert np.abs(d1.blocks[key] - d0.blocks[key]).max() <= 1e-12 * smax
+
+
 def test_eigh_wide_sector(ctx):
     """eigh of a sector wider than one work-group's LDS (k = 180): values
     against the dense eigvalsh within 1e-12 of the norm, A.U = U.Lambda."""
