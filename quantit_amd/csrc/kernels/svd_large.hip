@@ -246,6 +246,9 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
         // panel's earlier columns, so its latency overlaps that sum
         const double gpi = G[int64_t(p) * k + i];
         double acc = 0.0;
+        // unrolled so that the LDS reads of several terms are in flight at
+        // once; the FMA chain keeps its order
+#pragma unroll 8
         for (int t = 0; t < jj; ++t) acc = fma(col[t * k + i], col[t * k + p], acc);
         v = (gpi - acc) * isq;
         dg[i] -= v * v;
