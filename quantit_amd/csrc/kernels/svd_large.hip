@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
   __shared__ long long wkey[kLargeThreads / 64];
   const CholPanel P = ps[blockIdx.x];
   const int k = P.k;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double* G = base + P.g_off;
   double* L = base + P.l_off;
   double* Lp = base + P.lp_off;
@@ -225,7 +225,10 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
     if (lane == 0) wkey[wave] = key;
     __syncthreads();
     key = LLONG_MIN;
-    for (int w = 0; w < nwave; ++w) key = wkey[w] > key ? wkey[w] : key;
+    // the launch has kLargeThreads threads: a fixed trip count lets the
+    // sixteen reads be issued together
+#pragma unroll
+    for (int w = 0; w < kLargeThreads / 64; ++w) key = wkey[w] > key ? wkey[w] : key;
     const int p = key == LLONG_MIN ? -1 : 8191 - int(key & 0x1FFF);
     const double pv = p >= 0 ? dg[p] : 0.0;
     if (!(pv > 0.0)) {  // uniform: the rest is zero or negative noise
