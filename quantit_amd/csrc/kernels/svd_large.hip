@@ -185,7 +185,7 @@ __device__ __forceinline__ long long pc_dpp(long long key, int ctrl) {
 __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restrict__ base,
                                                                const CholPanel* __restrict__ ps, int j0, int nb) {
   extern __shared__ double lds[];
-  __shared__ long long wkey[kLargeThreads / 64];
+  __shared__ long long wkey[2][kLargeThreads / 64];  // wave maxima, by step parity
   const CholPanel P = ps[blockIdx.x];
   const int k = P.k;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -200,11 +200,32 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
     for (int64_t e = tid; e < int64_t(k) * nb; e += blockDim.x) Lp[e] = 0.0;
     return;
   }
-  for (int i = tid; i < k; i += blockDim.x) {
-    dg[i] = G[int64_t(i) * k + i];
-    used[i] = state[1 + i];
+  // the key of index i: the diagonal's top bits with the inverted index below
+  auto key_of = [&](int i) {
+    return used[i] ? LLONG_MIN : ((__double_as_longlong(dg[i]) & ~0x1FFFLL) | (8191 - i));
+  };
+  auto publish = [&](long long key, int slot) {
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+      const long long o = pc_dpp(key, st);
+      key = o > key ? o : key;
+    }
+    if (lane == 0) wkey[slot][wave] = key;
+  };
+  {
+    long long key = LLONG_MIN;
+    for (int i = tid; i < k; i += blockDim.x) {
+      dg[i] = G[int64_t(i) * k + i];
+      used[i] = state[1 + i];
+      const long long kk = key_of(i);
+      key = kk > key ? kk : key;
+    }
+    publish(key, 0);
   }
   __syncthreads();
+  // One barrier per step: a thread forms the next step's key from its own
+  // diagonal entries right after updating them, and the wave maxima go to
+  // the other parity's slots (last read before the previous barrier).
   int rank = k;
   for (int jj = 0; jj < nb; ++jj) {
     const int j = j0 + jj;
@@ -213,32 +234,20 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
       continue;
     }
     long long key = LLONG_MIN;
-    for (int i = tid; i < k; i += blockDim.x) {
-      const long long kk = used[i] ? LLONG_MIN : ((__double_as_longlong(dg[i]) & ~0x1FFFLL) | (8191 - i));
-      key = kk > key ? kk : key;
-    }
-#pragma unroll
-    for (int st = 0; st < 6; ++st) {
-      const long long o = pc_dpp(key, st);
-      key = o > key ? o : key;
-    }
-    if (lane == 0) wkey[wave] = key;
-    __syncthreads();
-    key = LLONG_MIN;
     // the launch has kLargeThreads threads: a fixed trip count lets the
     // sixteen reads be issued together
 #pragma unroll
-    for (int w = 0; w < kLargeThreads / 64; ++w) key = wkey[w] > key ? wkey[w] : key;
+    for (int w = 0; w < kLargeThreads / 64; ++w) key = wkey[jj & 1][w] > key ? wkey[jj & 1][w] : key;
     const int p = key == LLONG_MIN ? -1 : 8191 - int(key & 0x1FFF);
     const double pv = p >= 0 ? dg[p] : 0.0;
     if (!(pv > 0.0)) {  // uniform: the rest is zero or negative noise
       for (int t = jj; t < nb; ++t)
         for (int i = tid; i < k; i += blockDim.x) col[t * k + i] = 0.0;
       rank = j;
-      __syncthreads();
       break;
     }
     const double isq = fast_rsqrt_pc(pv), ljj = pv * isq;
+    long long nkey = LLONG_MIN;
     for (int i = tid; i < k; i += blockDim.x) {
       double v = 0.0;
       if (i == p) {
@@ -257,9 +266,13 @@ __global__ __launch_bounds__(kLargeThreads) void k_pchol_panel(double* __restric
         dg[i] -= v * v;
       }
       col[jj * k + i] = v;
+      const long long kk = key_of(i);
+      nkey = kk > nkey ? kk : nkey;
     }
+    publish(nkey, (jj + 1) & 1);
     __syncthreads();
   }
+  __syncthreads();  // the copies below read other threads' column entries
   for (int i = tid; i < k; i += blockDim.x) state[1 + i] = used[i];
   if (tid == 0 && rank < k) state[0] = rank;
   for (int64_t e = tid; e < int64_t(k) * nb; e += blockDim.x) {
