@@ -752,7 +752,7 @@ __host__ __device__ inline int gram_eig_gsz(int k, int threads) {
 }
 
 
-// ---- wave-local block Jacobi (k >= kWlMinK) ---------------------------------
+// ---- wave-local block Jacobi ---------------------------------------------------
 // The columns of X form NB = 2*ceil(k/16) blocks of 8 (columns >= k are zero
 // and never stored).  A work-group round pairs the blocks by the circle
 // ordering, one block pair (P, Q) per wave.  Inside the wave, 8-lane group g
@@ -763,8 +763,8 @@ __host__ __device__ inline int gram_eig_gsz(int k, int threads) {
 // round of a sweep also takes the 28 pairs inside each block, by xor pairing
 // of the groups (both groups of a pair compute the same rotation from the same
 // sums and update their own column).  One barrier per round instead of one per
-// column pair, and the columns cross LDS once per round.
-constexpr int kWlMinK = 48;
+// column pair, and the columns cross LDS once per round.  Each lane keeps R
+// rows of a column, R the smallest even count with 8 R >= k.
 // Nine waves: one per block pair at k = 144 (18 blocks), and at most three
 // waves per SIMD, so each keeps two 18-row columns and the shifted copy in
 // registers without spilling.
@@ -1186,56 +1186,25 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
   // ---- 3. one-sided Jacobi on the columns of X ------------------------------
   const double tol_rot = tol * (1.0 / 16.0);
   const int pi = tid / gsz, li = tid % gsz;
-  if (!ident && k >= kWlMinK) {
-    if (k <= 64)
+  if (!ident) {
+    if (k <= 16)
+      wl_sweeps<2>(X, k, tol_rot, max_sweeps, nrot);
+    else if (k <= 32)
+      wl_sweeps<4>(X, k, tol_rot, max_sweeps, nrot);
+    else if (k <= 48)
+      wl_sweeps<6>(X, k, tol_rot, max_sweeps, nrot);
+    else if (k <= 64)
       wl_sweeps<8>(X, k, tol_rot, max_sweeps, nrot);
+    else if (k <= 80)
+      wl_sweeps<10>(X, k, tol_rot, max_sweeps, nrot);
     else if (k <= 96)
       wl_sweeps<12>(X, k, tol_rot, max_sweeps, nrot);
+    else if (k <= 112)
+      wl_sweeps<14>(X, k, tol_rot, max_sweeps, nrot);
+    else if (k <= 128)
+      wl_sweeps<16>(X, k, tol_rot, max_sweeps, nrot);
     else
       wl_sweeps<18>(X, k, tol_rot, max_sweeps, nrot);
-  }
-  for (int sweep = 0; sweep < max_sweeps && !ident && k < kWlMinK; ++sweep) {
-    int* cnt = &nrot[sweep & 1];
-    for (int r = 0; r < ke - 1; ++r) {
-      if (pi < npairs) {
-        const int a0 = circle_block(pi, r, ke), b0 = circle_block(ke - 1 - pi, r, ke);
-        double* ya = X + min(a0, b0) * k;
-        double* yb = X + max(a0, b0) * k;
-        double xa[kEigR], xb[kEigR];
-#pragma unroll
-        for (int j = 0; j < kEigR; ++j) {
-          const int l = li + j * gsz;
-          xa[j] = l < k ? ya[l] : 0.0;
-          xb[j] = l < k ? yb[l] : 0.0;
-        }
-        double al = 0.0, be = 0.0, ga = 0.0;
-#pragma unroll
-        for (int j = 0; j < kEigR; ++j) {
-          al += xa[j] * xa[j];
-          be += xb[j] * xb[j];
-          ga += xa[j] * xb[j];
-        }
-        group_sum3(al, be, ga, gsz);
-        if (al > 0.0 && be > 0.0 && ga * ga > (tol_rot * al) * (tol_rot * be)) {
-          double c, sn;  // rotation annihilating ga
-          jacobi_cs(be - al, 2.0 * ga, c, sn);
-#pragma unroll
-          for (int j = 0; j < kEigR; ++j) {
-            const int l = li + j * gsz;
-            if (l < k) {
-              ya[l] = c * xa[j] - sn * xb[j];
-              yb[l] = sn * xa[j] + c * xb[j];
-            }
-          }
-          if (li == 0) atomicAdd(cnt, 1);
-        }
-      }
-      __syncthreads();
-    }
-    const int n = *cnt;
-    if (tid == 0) nrot[(sweep + 1) & 1] = 0;
-    __syncthreads();
-    if (n == 0) break;
   }
   if (!ident) {
     // ---- 4. normalise, complete zero columns --------------------------------
@@ -1358,9 +1327,11 @@ void gram_eig(hipStream_t st, double* base, const void* descs, int n, int* flags
                      static_cast<const SmallJacobi*>(descs), flags, tol, max_sweeps, precond);
 }
 
+// X, nrm, vec, hh and perm, and 16 doubles beyond them: the wave-local loads
+// of the last column read up to 8 R - k - 1 <= 15 rows past its end (masked).
 size_t gram_eig_lds_bytes(int k) {
   const int64_t ke = k + (k & 1);
-  return size_t(int64_t(k) * ke + 3 * ke) * sizeof(double) + size_t(ke) * sizeof(int);
+  return size_t(int64_t(k) * ke + 3 * ke + 16) * sizeof(double) + size_t(ke) * sizeof(int);
 }
 
 size_t jacobi_small_lds_limit() {
