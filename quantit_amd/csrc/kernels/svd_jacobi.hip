@@ -692,10 +692,6 @@ void jacobi_small(hipStream_t st, double* base, const void* descs, int n, int* f
 //    completed from the unit vector with the largest residual (two
 //    projection passes).
 // 4. W is written with the rows un-permuted (W = P Q).
-// Each column pair of a Jacobi round is owned by gsz lanes that keep its rows
-// in registers (kEigR per lane): one LDS read and one write per element.
-constexpr int kEigR = 18;
-
 // Sum over aligned groups of g lanes (g a power of two <= 16) with DPP moves:
 // xor 1 and xor 2 by quad permutes, then the 8- and 16-lane mirrors (after
 // the previous steps every lane of a half holds the same partial sum, so the
@@ -1007,8 +1003,7 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
                                                             int precond) {
   extern __shared__ double lds[];
   __shared__ int nrot[2];
-  __shared__ int nviol, nabs, best, piv;
-  __shared__ double pmax;
+  __shared__ int nviol, nabs, best;
   const SmallJacobi J = js[blockIdx.x];
   const int k = J.k;
   const int ke = k + (k & 1);
@@ -1023,7 +1018,7 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
   const int npairs = ke / 2;
   const int gsz = gram_eig_gsz(k, int(blockDim.x));
   if (k <= 0) return;
-  if (k > kGramEigMaxK || (k + gsz - 1) / gsz > kEigR) {  // never launched so by the host
+  if (k > kGramEigMaxK) {  // never launched so by the host
     for (int64_t e = tid; e < int64_t(k) * k; e += blockDim.x)
       Wg[(e / k) * J.ldw + e % k] = (e / k == e % k) ? 1.0 : 0.0;
     if (tid == 0 && J.flag_idx >= 0) flags[J.flag_idx] = -1;
@@ -1094,21 +1089,19 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
         const int i = tr * TR + a, l = tc * TC + b;
         g[a][b] = (owner && i < k && l < k) ? X[l * k + i] : 0.0;
       }
-    for (int i = tid; i < k; i += blockDim.x) {
-      vec[i] = X[i * k + i];
-      perm[i] = 0;
-    }
-    __syncthreads();  // every read of G from X precedes the L columns written below
-    // Pivot of the next step, by wave 0: the largest vec entry, smallest index
-    // on ties.  The key keeps the top 54 bits of the value and the inverted
-    // index in the low 10 bits (k <= 1023); pivoted entries hold -1, and
-    // non-positive values never win over positive ones under a signed compare.
-    auto pick = [&]() {
+    // Pivot keys: the top 54 bits of the remaining diagonal and the inverted
+    // index in the low 10 bits (k <= 1023), so the largest key is the largest
+    // value, smallest index on ties; pivoted entries hold -1, and non-positive
+    // values never win over positive ones under a signed compare.  Thread i <
+    // k keeps row i; each wave holding rows reduces its keys and publishes the
+    // maximum in wkey[parity of the step it serves], which every thread reads
+    // after the step's second barrier.
+    __shared__ long long wkey[2][kGramEigThreads / 64];
+    const int nwv = (k + 63) >> 6;  // waves that hold rows
+    auto publish = [&](int slot) {
       long long key = LLONG_MIN;
-      for (int i = tid; i < k; i += 64) {
-        const long long kk = (__double_as_longlong(vec[i]) & ~0x3FFLL) | (1023 - i);
-        key = kk > key ? kk : key;
-      }
+      if (tid < k) key = (__double_as_longlong(vec[tid]) & ~0x3FFLL) | (1023 - tid);
+#pragma unroll
       for (int st = 0; st < 4; ++st) {
         const long long o = __double_as_longlong(dpp_bits(__longlong_as_double(key), st));
         key = o > key ? o : key;
@@ -1117,26 +1110,31 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
       key = o > key ? o : key;
       o = __double_as_longlong(xor32(__longlong_as_double(key)));
       key = o > key ? o : key;
-      if (tid == 0) {
-        const int pi_ = key == LLONG_MIN ? -1 : 1023 - int(key & 0x3FF);
-        piv = pi_;
-        pmax = pi_ >= 0 ? vec[pi_] : 0.0;
-      }
+      if ((tid & 63) == 0) wkey[slot][tid >> 6] = key;
     };
-    if (tid < 64) pick();
-    __syncthreads();
-    // Each step: (1) the owners of column p copy it to hh; (2) thread i forms
-    // l_i, writes it to nrm and to column j of X and updates vec[i]; (3) the
-    // tiles take the rank-one update while wave 0 picks the next pivot.  The
-    // phases carry no chains of dependent LDS accesses.
+    if (tid < k) {
+      vec[tid] = X[tid * k + tid];
+      perm[tid] = 0;
+    }
+    if ((tid >> 6) < nwv) publish(0);
+    __syncthreads();  // every read of G from X precedes the L columns written below
+    // Each step: (1) every thread takes the pivot p from the wave maxima and
+    // the owners of column p copy it to hh; (2) thread i forms l_i, writes it
+    // to nrm and to column j of X, updates vec[i] and the waves publish the
+    // next keys; (3) the tiles take the rank-one update.  Two barriers a step:
+    // hh and nrm are written one phase after their last reads, behind a
+    // barrier, and the wave maxima alternate between two slots.
     int rk = k;
     for (int j = 0; j < k; ++j) {
-      if (!(pmax > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
+      long long top = wkey[j & 1][0];
+      for (int w = 1; w < nwv; ++w) top = wkey[j & 1][w] > top ? wkey[j & 1][w] : top;
+      const int p = top == LLONG_MIN ? -1 : 1023 - int(top & 0x3FF);
+      const double pmx = p >= 0 ? vec[p] : 0.0;
+      if (!(pmx > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
         rk = j;
         break;
       }
-      const int p = piv;
-      const double isq = fast_rsqrt(pmax), sq = pmax * isq;
+      const double isq = fast_rsqrt(pmx), sq = pmx * isq;
       if (owner && tc == p / TC) {
         // column p - tc TC of the tile by 0/1 weights: a select on the index
         // would be folded into an indexed load and move the tile to scratch
@@ -1152,7 +1150,8 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
         }
       }
       __syncthreads();
-      for (int i = tid; i < k; i += blockDim.x) {
+      if (tid < k) {
+        const int i = tid;
         const double l = i == p ? sq : (perm[i] ? 0.0 : hh[i] * isq);
         nrm[i] = l;
         X[j * k + i] = l;
@@ -1163,6 +1162,7 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
           vec[i] -= l * l;  // pivoted rows hold -1 and l = 0
         }
       }
+      if ((tid >> 6) < nwv) publish((j + 1) & 1);
       __syncthreads();
       if (owner) {
         double li[TR], lc[TC];
@@ -1175,8 +1175,6 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
 #pragma unroll
           for (int b = 0; b < TC; ++b) g[a][b] -= li[a] * lc[b];
       }
-      if (tid < 64) pick();
-      __syncthreads();
     }
     // columns from the rank on are zero; perm back to the identity
     for (int e = rk * k + tid; e < k * k; e += blockDim.x) X[e] = 0.0;
