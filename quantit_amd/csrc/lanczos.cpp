@@ -12,6 +12,7 @@
 // residual, beta) to decide whether to stop.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "dmrg.hpp"
 #include "kernels.hpp"
@@ -162,12 +163,13 @@ LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W, const BTens
   std::vector<char> present(lay.idx.size(), 1);  // blocks some Krylov vector so far has
   if (lay.n == 0) throw DomainError("two-site state without elements");
 
-  // scalars: c1[K], c2[K], nrm2, T[2K], out[3 + K]
+  // scalars: c1[K], c2[K], nrm2, T[2K], out[3 + K], then the NaN flag (an
+  // int32 in a double slot), so one read per step brings both back
   BufPtr sc = ctx->alloc(size_t(6 * K + 8) * sizeof(double));
   const int64_t s0 = sc->dptr() - ab;
   const int64_t c1 = s0, c2 = s0 + K, nrm2 = s0 + 2 * K, t_off = s0 + 2 * K + 1, out = s0 + 4 * K + 1;
-  BufPtr flag = ctx->alloc(sizeof(int32_t));
-  kern::fill_i32(ctx->stream(), reinterpret_cast<int32_t*>(flag->cptr()), 1, 0);
+  int32_t* d_flag = reinterpret_cast<int32_t*>(ab + out + 3 + K);
+  kern::fill_i32(ctx->stream(), d_flag, 1, 0);
   BufPtr V, w, part;
   auto alloc_vectors = [&](int64_t n) {
     V = ctx->alloc(size_t(K) * size_t(n) * sizeof(double));
@@ -286,13 +288,14 @@ LanczosResult lanczos_ground(const BTensor& theta, const BTensor& W, const BTens
     }
     {
       KernelTimer kt(*ctx, "lanczos_tridiag");
-      kern::lanczos_tridiag(ctx->stream(), ab, t_off, K, j, c1, c2, nrm2, out,
-                            reinterpret_cast<int*>(flag->cptr()));
+      kern::lanczos_tridiag(ctx->stream(), ab, t_off, K, j, c1, c2, nrm2, out, d_flag);
     }
+    double host_all[3 + kern::kMaxKrylov + 1];
+    ctx->read_small(host_all, ab + out, size_t(3 + K + 1) * sizeof(double));
     int32_t hflag = 0;
-    ctx->read_small(host_out, ab + out, sizeof(host_out));
-    ctx->read_small(&hflag, flag->cptr(), sizeof(int32_t));
+    std::memcpy(&hflag, host_all + 3 + K, sizeof(int32_t));
     if (hflag) throw LogicError("nan found in output tensor");
+    std::copy(host_all, host_all + 3, host_out);
     const double lam = host_out[0], resid = host_out[1], beta = host_out[2];
     m = j + 1;
     const bool done = resid <= tol * std::fabs(lam) || !(beta > 1e-15 * std::fabs(lam)) || m == K;

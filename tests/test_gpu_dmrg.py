@@ -166,6 +166,23 @@ def test_dmrg_lanczos_matches_oracle_and_exact(ctx):
     assert abs(E - e0) <= 1e-10 * abs(e0)
 
 
+def test_lanczos_nan_in_state_raises(ctx):
+    """A NaN in the two-site state reaches the k-step solve's tridiagonal
+    step, whose flag comes back with the step's scalars: the update fails
+    with status 4 (std::logic_error), as the one-step form's NaN check
+    (sources/dmrg.cpp:566-569) makes it fail."""
+    import quantit_amd as qa
+    mpo, mps = _setup(dor.xxz_dense(1.0, 6), seed=2)
+    key = sorted(mps[1].blocks)[0]
+    blk = mps[1].blocks[key].copy()
+    blk.flat[0] = np.nan
+    mps[1].blocks[key] = blk
+    o = dict(cutoff=1e-10, convergence_criterion=0.0, maximum_bond=16, minimum_bond=4, maximum_iterations=1)
+    with _lanczos_ctx(ctx, 8, 1e-12):
+        with pytest.raises(RuntimeError, match="nan"):
+            qa.dmrg([to_dev(ctx, t) for t in mpo], [to_dev(ctx, t) for t in mps], qa.DmrgOptions(**o))
+
+
 def test_lanczos_chain_L32_D128_converges(ctx):
     """The L=32, D=128 chain in k-step mode reaches |dE/E| <= 1e-10 between
     sweeps within 6 sweeps, below the one-step trajectory's sixth-sweep
