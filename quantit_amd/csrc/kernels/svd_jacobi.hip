@@ -997,6 +997,75 @@ __device__ __forceinline__ void wl_sweeps(double* X, int k, double tol_rot, int 
   }
 }
 
+// Pivoted Cholesky of G (in X, k x k column-major, k <= KM <= 64) by one
+// wave: lane i keeps row i of the trailing matrix in registers, so a step
+// takes no work-group barrier.  The trailing matrix stays symmetric to the
+// bit (both halves take the same products), so the pivot column is lane p's
+// row, which lane p writes to LDS (every index static: an index chosen at
+// run time would move the row to scratch).  Pivot rule, keys and arithmetic
+// are those of the register-tile form (the largest remaining diagonal,
+// smallest index on ties; l = G(:, p) / sqrt(G(p, p)); every update the same
+// fused multiply-add), so L is the same to the bit; column j of X receives l
+// in the original row order.  Returns the rank.
+template <int KM>
+__device__ __forceinline__ int chol_wave(double* X, int k) {
+  __shared__ double lb[64];  // the step's column l, one entry per lane
+  __shared__ double cp[64];  // the pivot's row of the trailing matrix
+  const int lane = threadIdx.x & 63;
+  const bool row = lane < k;
+  double g[KM];
+#pragma unroll
+  for (int c = 0; c < KM; ++c) g[c] = (row && c < k) ? X[c * k + lane] : 0.0;
+  double dg = row ? X[lane * k + lane] : 0.0;
+  bool used = !row;
+  int rk = k;
+  for (int j = 0; j < k; ++j) {
+    long long key = used ? LLONG_MIN : ((__double_as_longlong(dg) & ~0x3FFLL) | (1023 - lane));
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const long long o = __double_as_longlong(dpp_bits(__longlong_as_double(key), st));
+      key = o > key ? o : key;
+    }
+    long long o = __double_as_longlong(swizzle64<0x401F>(__longlong_as_double(key)));
+    key = o > key ? o : key;
+    o = __double_as_longlong(xor32(__longlong_as_double(key)));
+    key = o > key ? o : key;
+    const int p = __builtin_amdgcn_readfirstlane(key == LLONG_MIN ? -1 : 1023 - int(key & 0x3FF));
+    double pv = 0.0;
+    if (p >= 0) {
+      const long long b = __double_as_longlong(dg);
+      const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffffLL), p);
+      const int hi = __builtin_amdgcn_readlane(int(b >> 32), p);
+      pv = __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+    }
+    if (!(pv > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
+      rk = j;
+      break;
+    }
+    const double isq = fast_rsqrt(pv), sq = pv * isq;
+    if (lane == p) {
+#pragma unroll
+      for (int c = 0; c < KM; ++c) cp[c] = g[c];
+    }
+    __builtin_amdgcn_wave_barrier();  // one wave: its LDS accesses complete in order
+    const double gp = cp[lane];
+    const double l = lane == p ? sq : (used ? 0.0 : gp * isq);
+    if (lane == p) {
+      used = true;
+      dg = -1.0;
+    } else {
+      dg -= l * l;  // pivoted rows hold -1 and l = 0
+    }
+    if (row) X[j * k + lane] = l;
+    lb[lane] = l;
+    __builtin_amdgcn_wave_barrier();  // one wave: its LDS accesses complete in order
+#pragma unroll
+    for (int c = 0; c < KM; ++c) g[c] -= l * lb[c];
+    __builtin_amdgcn_wave_barrier();  // every read of lb precedes the next step's writes
+  }
+  return rk;
+}
+
 __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict__ base,
                                                             const SmallJacobi* __restrict__ js,
                                                             int* __restrict__ flags, double tol, int max_sweeps,
@@ -1070,110 +1139,121 @@ __global__ __launch_bounds__(kGramEigThreads) void k_gram_eig(double* __restrict
   const bool ident = nviol == 0 || k < 2;
   if (!ident && precond) {
     // ---- 2. pivoted Cholesky, G held in registers ------------------------------
-    // Thread t owns an 8 x 5 tile of G.  Step j picks the largest remaining
-    // diagonal p (smallest index on ties), the owners of column p publish
-    // l = G(:, p) / sqrt(G(p, p)) (zero on rows already pivoted), every tile
-    // takes the rank-one update and vec keeps the remaining diagonal.  Column
-    // j of X receives l in the original row order, so X X^T = G and the
-    // Jacobi below needs no permutation; perm flags the pivoted rows here and
-    // is reset to the identity afterwards.
-    constexpr int TR = 8, TC = 5;
-    const int ntc = (k + TC - 1) / TC;
-    const bool owner = tid < ((k + TR - 1) / TR) * ntc;
-    const int tr = owner ? tid / ntc : 0, tc = owner ? tid % ntc : 0;
-    double g[TR][TC];
-#pragma unroll
-    for (int a = 0; a < TR; ++a)
-#pragma unroll
-      for (int b = 0; b < TC; ++b) {
-        const int i = tr * TR + a, l = tc * TC + b;
-        g[a][b] = (owner && i < k && l < k) ? X[l * k + i] : 0.0;
-      }
-    // Pivot keys: the top 54 bits of the remaining diagonal and the inverted
-    // index in the low 10 bits (k <= 1023), so the largest key is the largest
-    // value, smallest index on ties; pivoted entries hold -1, and non-positive
-    // values never win over positive ones under a signed compare.  Thread i <
-    // k keeps row i; each wave holding rows reduces its keys and publishes the
-    // maximum in wkey[parity of the step it serves], which every thread reads
-    // after the step's second barrier.
-    __shared__ long long wkey[2][kGramEigThreads / 64];
-    const int nwv = (k + 63) >> 6;  // waves that hold rows
-    auto publish = [&](int slot) {
-      long long key = LLONG_MIN;
-      if (tid < k) key = (__double_as_longlong(vec[tid]) & ~0x3FFLL) | (1023 - tid);
-#pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        const long long o = __double_as_longlong(dpp_bits(__longlong_as_double(key), st));
-        key = o > key ? o : key;
-      }
-      long long o = __double_as_longlong(swizzle64<0x401F>(__longlong_as_double(key)));
-      key = o > key ? o : key;
-      o = __double_as_longlong(xor32(__longlong_as_double(key)));
-      key = o > key ? o : key;
-      if ((tid & 63) == 0) wkey[slot][tid >> 6] = key;
-    };
-    if (tid < k) {
-      vec[tid] = X[tid * k + tid];
-      perm[tid] = 0;
-    }
-    if ((tid >> 6) < nwv) publish(0);
-    __syncthreads();  // every read of G from X precedes the L columns written below
-    // Each step: (1) every thread takes the pivot p from the wave maxima and
-    // the owners of column p copy it to hh; (2) thread i forms l_i, writes it
-    // to nrm and to column j of X, updates vec[i] and the waves publish the
-    // next keys; (3) the tiles take the rank-one update.  Two barriers a step:
-    // hh and nrm are written one phase after their last reads, behind a
-    // barrier, and the wave maxima alternate between two slots.
+    // Up to k = 64 one wave does it (chol_wave).  Wider, thread t owns an 8 x 5
+    // tile of G.  Step j picks the largest remaining diagonal p (smallest
+    // index on ties), the owners of column p publish l = G(:, p) / sqrt(G(p,
+    // p)) (zero on rows already pivoted), every tile takes the rank-one update
+    // and vec keeps the remaining diagonal.  Column j of X receives l in the
+    // original row order, so X X^T = G and the Jacobi below needs no
+    // permutation; perm flags the pivoted rows here and is reset to the
+    // identity afterwards.
+    __shared__ int rk_sh;
     int rk = k;
-    for (int j = 0; j < k; ++j) {
-      long long top = wkey[j & 1][0];
-      for (int w = 1; w < nwv; ++w) top = wkey[j & 1][w] > top ? wkey[j & 1][w] : top;
-      const int p = top == LLONG_MIN ? -1 : 1023 - int(top & 0x3FF);
-      const double pmx = p >= 0 ? vec[p] : 0.0;
-      if (!(pmx > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
-        rk = j;
-        break;
-      }
-      const double isq = fast_rsqrt(pmx), sq = pmx * isq;
-      if (owner && tc == p / TC) {
-        // column p - tc TC of the tile by 0/1 weights: a select on the index
-        // would be folded into an indexed load and move the tile to scratch
-        double m[TC];
-#pragma unroll
-        for (int b = 0; b < TC; ++b) m[b] = (p - tc * TC == b) ? 1.0 : 0.0;
-#pragma unroll
-        for (int a = 0; a < TR; ++a) {
-          double v = 0.0;
-#pragma unroll
-          for (int b = 0; b < TC; ++b) v += g[a][b] * m[b];
-          if (tr * TR + a < k) hh[tr * TR + a] = v;
-        }
+    if (k <= 64) {
+      if (tid < 64) {
+        const int r = k <= 32 ? chol_wave<32>(X, k) : chol_wave<64>(X, k);
+        if (tid == 0) rk_sh = r;
       }
       __syncthreads();
+      rk = rk_sh;
+    } else {
+      constexpr int TR = 8, TC = 5;
+      const int ntc = (k + TC - 1) / TC;
+      const bool owner = tid < ((k + TR - 1) / TR) * ntc;
+      const int tr = owner ? tid / ntc : 0, tc = owner ? tid % ntc : 0;
+      double g[TR][TC];
+#pragma unroll
+      for (int a = 0; a < TR; ++a)
+#pragma unroll
+        for (int b = 0; b < TC; ++b) {
+          const int i = tr * TR + a, l = tc * TC + b;
+          g[a][b] = (owner && i < k && l < k) ? X[l * k + i] : 0.0;
+        }
+      // Pivot keys: the top 54 bits of the remaining diagonal and the inverted
+      // index in the low 10 bits (k <= 1023), so the largest key is the largest
+      // value, smallest index on ties; pivoted entries hold -1, and non-positive
+      // values never win over positive ones under a signed compare.  Thread i <
+      // k keeps row i; each wave holding rows reduces its keys and publishes the
+      // maximum in wkey[parity of the step it serves], which every thread reads
+      // after the step's second barrier.
+      __shared__ long long wkey[2][kGramEigThreads / 64];
+      const int nwv = (k + 63) >> 6;  // waves that hold rows
+      auto publish = [&](int slot) {
+        long long key = LLONG_MIN;
+        if (tid < k) key = (__double_as_longlong(vec[tid]) & ~0x3FFLL) | (1023 - tid);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          const long long o = __double_as_longlong(dpp_bits(__longlong_as_double(key), st));
+          key = o > key ? o : key;
+        }
+        long long o = __double_as_longlong(swizzle64<0x401F>(__longlong_as_double(key)));
+        key = o > key ? o : key;
+        o = __double_as_longlong(xor32(__longlong_as_double(key)));
+        key = o > key ? o : key;
+        if ((tid & 63) == 0) wkey[slot][tid >> 6] = key;
+      };
       if (tid < k) {
-        const int i = tid;
-        const double l = i == p ? sq : (perm[i] ? 0.0 : hh[i] * isq);
-        nrm[i] = l;
-        X[j * k + i] = l;
-        if (i == p) {
-          vec[p] = -1.0;
-          perm[p] = 1;
-        } else {
-          vec[i] -= l * l;  // pivoted rows hold -1 and l = 0
-        }
+        vec[tid] = X[tid * k + tid];
+        perm[tid] = 0;
       }
-      if ((tid >> 6) < nwv) publish((j + 1) & 1);
-      __syncthreads();
-      if (owner) {
-        double li[TR], lc[TC];
+      if ((tid >> 6) < nwv) publish(0);
+      __syncthreads();  // every read of G from X precedes the L columns written below
+      // Each step: (1) every thread takes the pivot p from the wave maxima and
+      // the owners of column p copy it to hh; (2) thread i forms l_i, writes it
+      // to nrm and to column j of X, updates vec[i] and the waves publish the
+      // next keys; (3) the tiles take the rank-one update.  Two barriers a step:
+      // hh and nrm are written one phase after their last reads, behind a
+      // barrier, and the wave maxima alternate between two slots.
+      for (int j = 0; j < k; ++j) {
+        long long top = wkey[j & 1][0];
+        for (int w = 1; w < nwv; ++w) top = wkey[j & 1][w] > top ? wkey[j & 1][w] : top;
+        const int p = top == LLONG_MIN ? -1 : 1023 - int(top & 0x3FF);
+        const double pmx = p >= 0 ? vec[p] : 0.0;
+        if (!(pmx > 0.0)) {  // uniform: the rest of the matrix is zero or negative noise
+          rk = j;
+          break;
+        }
+        const double isq = fast_rsqrt(pmx), sq = pmx * isq;
+        if (owner && tc == p / TC) {
+          // column p - tc TC of the tile by 0/1 weights: a select on the index
+          // would be folded into an indexed load and move the tile to scratch
+          double m[TC];
 #pragma unroll
-        for (int a = 0; a < TR; ++a) li[a] = tr * TR + a < k ? nrm[tr * TR + a] : 0.0;
+          for (int b = 0; b < TC; ++b) m[b] = (p - tc * TC == b) ? 1.0 : 0.0;
 #pragma unroll
-        for (int b = 0; b < TC; ++b) lc[b] = tc * TC + b < k ? nrm[tc * TC + b] : 0.0;
+          for (int a = 0; a < TR; ++a) {
+            double v = 0.0;
 #pragma unroll
-        for (int a = 0; a < TR; ++a)
+            for (int b = 0; b < TC; ++b) v += g[a][b] * m[b];
+            if (tr * TR + a < k) hh[tr * TR + a] = v;
+          }
+        }
+        __syncthreads();
+        if (tid < k) {
+          const int i = tid;
+          const double l = i == p ? sq : (perm[i] ? 0.0 : hh[i] * isq);
+          nrm[i] = l;
+          X[j * k + i] = l;
+          if (i == p) {
+            vec[p] = -1.0;
+            perm[p] = 1;
+          } else {
+            vec[i] -= l * l;  // pivoted rows hold -1 and l = 0
+          }
+        }
+        if ((tid >> 6) < nwv) publish((j + 1) & 1);
+        __syncthreads();
+        if (owner) {
+          double li[TR], lc[TC];
 #pragma unroll
-          for (int b = 0; b < TC; ++b) g[a][b] -= li[a] * lc[b];
+          for (int a = 0; a < TR; ++a) li[a] = tr * TR + a < k ? nrm[tr * TR + a] : 0.0;
+#pragma unroll
+          for (int b = 0; b < TC; ++b) lc[b] = tc * TC + b < k ? nrm[tc * TC + b] : 0.0;
+#pragma unroll
+          for (int a = 0; a < TR; ++a)
+#pragma unroll
+            for (int b = 0; b < TC; ++b) g[a][b] -= li[a] * lc[b];
+        }
       }
     }
     // columns from the rank on are zero; perm back to the identity
