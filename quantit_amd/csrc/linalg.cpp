@@ -182,6 +182,10 @@ int jacobi_sweeps(Context* ctx, double* ab, const std::vector<kern::SvdSector>& 
     max_ell = std::max(max_ell, sc.ell);
     for (int pi = 0; pi < sc.nb / 2; ++pi) work.push_back({sc.y_off, sc.w_off, sc.ell, sc.nb, sc.flag_idx, pi});
   }
+  // longest rows first: when a round has more work-groups than the device
+  // holds at once, the long ones start in the first wave of work-groups
+  std::stable_sort(work.begin(), work.end(),
+                   [](const kern::RoundWork& a, const kern::RoundWork& b) { return a.ell > b.ell; });
   const kern::RoundWork* d_work = work.empty() ? nullptr : ctx->upload_vec(work);
   auto enqueue = [&](int sweep) {
     int max_nb = 0;
@@ -710,6 +714,10 @@ Batched run_batched(const BTensor& t, const std::vector<SectorPlan>& plans, bool
           const auto& s = B.secs[wide[q]];
           jd.push_back({gb + go[q], gb + gwo[q], s.k, s.k, s.k, s.k, q, nw + q});
         }
+        // widest first, so that the longest solves start in the first wave of
+        // work-groups when there are more sectors than the device holds
+        std::stable_sort(jd.begin(), jd.end(),
+                         [](const kern::SmallJacobi& a, const kern::SmallJacobi& b) { return a.k > b.k; });
         static const char* const kEigNames[] = {"svd_eig", "svd_eig_it1", "svd_eig_it2", "svd_eig_it3+"};
         const auto* d_jd = ctx->upload_vec(jd);
         if (overlap) ctx->fork();
